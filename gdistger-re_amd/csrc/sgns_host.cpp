@@ -1,0 +1,682 @@
+// Host layer of the SGNS trainer: the C ABI of include/dger_sgns.h.
+//
+// Reference boundary replaced: dsgl() (include/dsgl.hpp:1464-1584) with its
+// preparatory_work() (dsgl.hpp:511-519) and Train_SGNS_MPI()
+// (dsgl.hpp:522-1427).  Host work here is the O(V) part the reference also
+// does serially (vocabulary sort, sigmoid table, the per-rank boundaries of
+// the noise table); every O(corpus) or O(V*D) pass runs on the GPU.
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dger_sgns.h"
+#include "sgns_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define DGER_HIP(expr)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(DGER_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+// Device allocation that frees itself.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, n ? n : 1);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// dsgl.hpp:1429-1441: index of the exact-match flag, or -1.
+int arg_pos(const char* flag, int argc, char** argv) {
+  for (int a = 1; a < argc; a++)
+    if (argv[a] && !std::strcmp(flag, argv[a])) return a;
+  return -1;
+}
+
+// word2vec sigmoid table, dsgl.hpp:1557-1562 (float exp of a float argument).
+void build_exp_table(float* t) {
+  for (int i = 0; i < dger::kExpTable; i++) {
+    float e = std::exp((i / (float)1000 * 2 - 1) * 6);
+    t[i] = e / (e + 1);
+  }
+}
+
+// sum(cn^0.75) in double over `threads` static chunks combined in order
+// (dsgl.hpp:125-132; OpenMP schedule(static) split).
+double pow_sum(const std::vector<int32_t>& cn_rank, int threads) {
+  if (threads < 1) threads = 1;
+  const int64_t n = (int64_t)cn_rank.size();
+  const int64_t q = n / threads, t = n % threads;
+  double total = 0.;
+  for (int th = 0; th < threads; th++) {
+    int64_t qq = q, tt = t;
+    if (th < tt) { tt = 0; qq++; }
+    const int64_t s0 = qq * th + tt, e0 = s0 + qq;
+    double part = 0.;
+    for (int64_t i = s0; i < e0; i++) part += std::pow((double)cn_rank[i], (double)0.75f);
+    total += part;
+  }
+  return total;
+}
+
+// Rank boundaries of the lagged-CDF noise table (dsgl.hpp:134-146).  The
+// reference advances at most one rank per slot: the k-th advance happens at
+// the first slot a after the previous advance with (float)a/(float)T > d1,
+// where d1 is the float running sum.  starts[k] = first slot holding rank k.
+std::vector<int64_t> table_starts(const std::vector<int32_t>& cn_rank, int64_t T, int threads) {
+  const int V = (int)cn_rank.size();
+  const double pw = pow_sum(cn_rank, threads);
+  const float Tf = (float)T;
+  auto first_above = [&](float d) -> int64_t {
+    int64_t lo = 0, hi = T;
+    while (lo < hi) {
+      const int64_t mid = lo + (hi - lo) / 2;
+      if ((float)mid / Tf > d) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+  };
+  std::vector<int64_t> starts;
+  starts.reserve(V);
+  starts.push_back(0);
+  float d1 = (float)(std::pow((double)cn_rank[0], (double)0.75f) / pw);
+  int64_t prev = -1;
+  for (int k = 1; k <= V - 1; k++) {
+    const int64_t a = std::max(prev + 1, first_above(d1));
+    if (a >= T) break;
+    starts.push_back(a + 1);
+    prev = a;
+    d1 = (float)((double)d1 + std::pow((double)cn_rank[k], (double)0.75f) / pw);
+  }
+  return starts;
+}
+
+size_t row_bytes(int precision) { return precision == DGER_PRECISION_BF16 ? 2 : 4; }
+
+}  // namespace
+
+struct dger_ctx {
+  dger_sgns_opts o;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t V = 0;
+  int D = 0;
+  uint64_t train_words = 0;
+  std::vector<int32_t> cn, rank_id, id_rank;
+  DevBuf id_rank_d, keep_d, table_d, exp_d, wih_d, woh_d, corpus_d, scratch_d, counters_d,
+      wsteps_d, walpha_d, rec_d, tmp_d;
+  uint64_t n_tokens = 0;
+  int Lmax = 0;
+  bool have_corpus = false;
+
+  ~dger_ctx() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+const char* dger_last_error(void) { return g_err.c_str(); }
+
+int32_t dger_record_stride(int32_t window, int32_t negative) { return dger::record_stride(window, negative); }
+
+int dger_sgns_default_opts(dger_sgns_opts* o) {
+  if (!o) return fail(DGER_E_ARG, "null opts");
+  std::memset(o, 0, sizeof(*o));
+  // dsgl.hpp:67-76
+  o->size = 100;
+  o->window = 5;
+  o->negative = 5;
+  o->batch_size = 11;
+  o->sample = 1e-3f;
+  o->alpha = 0.1f;
+  o->min_count = 5;
+  o->iter = 5;
+  o->threads = 12;
+  o->debug = 2;
+  o->binary = 0;
+  o->sync_period = 0.1f;
+  o->min_sync_words = 1024;
+  o->full_sync_times = 0;
+  o->message_size = 1024;
+  o->workers = 0;
+  o->precision = DGER_PRECISION_FP32;
+  o->table_size = 100000000LL;
+  o->device = 0;
+  return DGER_OK;
+}
+
+int dger_sgns_parse_args(int argc, char** argv, dger_sgns_opts* o) {
+  int rc = dger_sgns_default_opts(o);
+  if (rc) return rc;
+  int i;
+  auto val = [&](const char* flag) -> const char* {
+    i = arg_pos(flag, argc, argv);
+    if (i < 0) return nullptr;
+    if (i + 1 >= argc || !argv[i + 1]) return "";
+    return argv[i + 1];
+  };
+  auto missing = [&](const char* flag) { return fail(DGER_E_ARG, std::string("missing value for ") + flag); };
+  auto copy_str = [&](char* dst, const char* v) {
+    std::strncpy(dst, v, 511);
+    dst[511] = 0;
+  };
+  const char* v;
+#define DGER_INT(flag, field)                        \
+  if ((v = val(flag))) {                             \
+    if (!*v) return missing(flag);                   \
+    o->field = (int32_t)std::atoi(v);                \
+  }
+#define DGER_FLT(flag, field)                        \
+  if ((v = val(flag))) {                             \
+    if (!*v) return missing(flag);                   \
+    o->field = (float)std::atof(v);                  \
+  }
+#define DGER_STR(flag, field)                        \
+  if ((v = val(flag))) {                             \
+    if (!*v) return missing(flag);                   \
+    copy_str(o->field, v);                           \
+  }
+  DGER_INT("-size", size)
+  DGER_STR("-save-vocab", save_vocab)
+  DGER_STR("-read-vocab", read_vocab)
+  DGER_INT("-debug", debug)
+  DGER_INT("-binary", binary)
+  DGER_FLT("-alpha", alpha)
+  DGER_STR("-eoutput", eoutput)
+  DGER_INT("-window", window)
+  DGER_FLT("-sample", sample)
+  DGER_INT("-negative", negative)
+  DGER_INT("-threads", threads)
+  DGER_INT("-iter", iter)
+  DGER_INT("-min-count", min_count)
+  DGER_INT("-batch-size", batch_size)
+  if (arg_pos("-disk", argc, argv) > 0) o->disk = 1;
+  DGER_FLT("-sync-period", sync_period)
+  DGER_INT("-min-sync-words", min_sync_words)
+  DGER_INT("-full-sync-times", full_sync_times)
+  DGER_INT("-message-size", message_size)
+  DGER_STR("-save-emb", save_emb)
+  DGER_INT("-workers", workers)
+  DGER_INT("-device", device)
+  if ((v = val("-precision"))) {
+    if (!*v) return missing("-precision");
+    if (!std::strcmp(v, "bf16") || !std::strcmp(v, "1")) o->precision = DGER_PRECISION_BF16;
+    else if (!std::strcmp(v, "fp32") || !std::strcmp(v, "0")) o->precision = DGER_PRECISION_FP32;
+    else return fail(DGER_E_ARG, std::string("unknown -precision ") + v);
+  }
+  if ((v = val("-table-size"))) {
+    if (!*v) return missing("-table-size");
+    o->table_size = std::atoll(v);
+  }
+#undef DGER_INT
+#undef DGER_FLT
+#undef DGER_STR
+  return DGER_OK;
+}
+
+static int check_opts(const dger_sgns_opts* o) {
+  if (o->size < 2 || o->size > 1024 || (o->size & 1))
+    return fail(DGER_E_UNSUPPORTED, "-size must be even and in [2, 1024]");
+  if (o->window < 1 || o->window > 64) return fail(DGER_E_UNSUPPORTED, "-window must be in [1, 64]");
+  if (o->negative < 0 || o->negative > 62) return fail(DGER_E_UNSUPPORTED, "-negative must be in [0, 62]");
+  if (o->iter < 1) return fail(DGER_E_ARG, "-iter must be >= 1");
+  if (o->table_size < 1) return fail(DGER_E_ARG, "-table-size must be >= 1");
+  if (o->precision != DGER_PRECISION_FP32 && o->precision != DGER_PRECISION_BF16)
+    return fail(DGER_E_ARG, "unknown precision");
+  if (o->workers < 0) return fail(DGER_E_ARG, "-workers must be >= 0");
+  return DGER_OK;
+}
+
+int dger_ctx_reset_model(dger_ctx* c) {
+  if (!c) return fail(DGER_E_ARG, "null ctx");
+  const uint64_t n = (uint64_t)c->V * (uint64_t)c->D;
+  if (c->o.precision == DGER_PRECISION_BF16) {
+    DGER_HIP(c->tmp_d.ensure(n * 4));
+    DGER_HIP(dger::launch_init_net(c->tmp_d.as<float>(), n, c->D, c->stream));
+    DGER_HIP(dger::launch_f32_to_bf16(c->tmp_d.as<float>(), c->wih_d.as<uint16_t>(), n, c->stream));
+  } else {
+    DGER_HIP(dger::launch_init_net(c->wih_d.as<float>(), n, c->D, c->stream));
+  }
+  DGER_HIP(hipMemsetAsync(c->woh_d.p, 0, n * row_bytes(c->o.precision), c->stream));
+  DGER_HIP(hipStreamSynchronize(c->stream));
+  return DGER_OK;
+}
+
+int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32_t v_num, dger_ctx** out) {
+  if (!opts || !vertex_cn || !out) return fail(DGER_E_ARG, "null argument");
+  *out = nullptr;
+  int rc = check_opts(opts);
+  if (rc) return rc;
+  if (v_num < 2) return fail(DGER_E_VOCAB, "vocabulary needs at least 2 vertices");
+  for (uint32_t i = 0; i < v_num; i++)
+    if (vertex_cn[i] < 0) return fail(DGER_E_ARG, "negative vertex count");
+  int ndev = 0;
+  DGER_HIP(hipGetDeviceCount(&ndev));
+  if (opts->device < 0 || opts->device >= ndev) return fail(DGER_E_ARG, "device ordinal out of range");
+  DGER_HIP(hipSetDevice(opts->device));
+
+  dger_ctx* c = new dger_ctx();
+  c->o = *opts;
+  c->V = v_num;
+  c->D = opts->size;
+  auto bail = [&](int code) { delete c; return code; };
+  if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+      hipEventCreate(&c->ev1) != hipSuccess)
+    return bail(fail(DGER_E_HIP, "stream/event creation failed"));
+
+  // Vocabulary (dsgl.hpp:314-382): ranks by count, std::sort with the
+  // reference's comparator on (cn, id) records in id order.
+  struct Rec { int cn; uint32_t id; };
+  std::vector<Rec> v(v_num);
+  c->cn.assign(vertex_cn, vertex_cn + v_num);
+  c->train_words = 0;
+  for (uint32_t i = 0; i < v_num; i++) {
+    v[i].cn = vertex_cn[i];
+    v[i].id = i;
+    c->train_words += (uint64_t)(int64_t)vertex_cn[i];
+  }
+  std::sort(v.begin(), v.end(), [](Rec a, Rec b) { return a.cn > b.cn; });
+  c->rank_id.resize(v_num);
+  c->id_rank.resize(v_num);
+  std::vector<int32_t> cn_rank(v_num);
+  for (uint32_t r = 0; r < v_num; r++) {
+    c->rank_id[r] = (int32_t)v[r].id;
+    c->id_rank[v[r].id] = (int32_t)r;
+    cn_rank[r] = v[r].cn;
+  }
+  // Subsampling thresholds per rank; cn_vocab is indexed by the rank as in
+  // dsgl.hpp:923.
+  std::vector<float> keep(v_num);
+  for (uint32_t w = 0; w < v_num; w++) {
+    const float ratio = (opts->sample * (float)c->train_words) / (float)vertex_cn[w];
+    keep[w] = std::sqrt(ratio) + ratio;
+  }
+  float exp_table[dger::kExpTable];
+  build_exp_table(exp_table);
+  const std::vector<int64_t> starts = table_starts(cn_rank, opts->table_size, opts->threads);
+
+  const uint64_t n = (uint64_t)v_num * (uint64_t)c->D;
+  const size_t rb = row_bytes(opts->precision);
+  if (c->id_rank_d.ensure(v_num * 4) || c->keep_d.ensure(v_num * 4) ||
+      c->table_d.ensure((size_t)opts->table_size * 4) || c->exp_d.ensure(sizeof(exp_table)) ||
+      c->wih_d.ensure(n * rb) || c->woh_d.ensure(n * rb) ||
+      c->counters_d.ensure(dger::kNumCounters * 8) || c->tmp_d.ensure(starts.size() * 8))
+    return bail(fail(DGER_E_ALLOC, "device allocation failed"));
+  if (hipMemcpyAsync(c->id_rank_d.p, c->id_rank.data(), v_num * 4, hipMemcpyHostToDevice, c->stream) ||
+      hipMemcpyAsync(c->keep_d.p, keep.data(), v_num * 4, hipMemcpyHostToDevice, c->stream) ||
+      hipMemcpyAsync(c->exp_d.p, exp_table, sizeof(exp_table), hipMemcpyHostToDevice, c->stream) ||
+      hipMemcpyAsync(c->tmp_d.p, starts.data(), starts.size() * 8, hipMemcpyHostToDevice, c->stream))
+    return bail(fail(DGER_E_HIP, "upload failed"));
+  if (dger::launch_table_fill(c->table_d.as<int32_t>(), (uint64_t)opts->table_size, c->tmp_d.as<int64_t>(),
+                              (int)starts.size(), c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return bail(fail(DGER_E_HIP, "noise table fill failed"));
+  rc = dger_ctx_reset_model(c);
+  if (rc) return bail(rc);
+  *out = c;
+  return DGER_OK;
+}
+
+int dger_ctx_set_corpus(dger_ctx* c, const int32_t* corpus, uint64_t n_tokens) {
+  if (!c || (!corpus && n_tokens)) return fail(DGER_E_ARG, "null argument");
+  int run = 0, lmax = 0;
+  for (uint64_t i = 0; i < n_tokens; i++) {
+    const int32_t id = corpus[i];
+    if (id < -1 || id >= (int64_t)c->V) return fail(DGER_E_CORPUS, "corpus id out of range at " + std::to_string(i));
+    if (id == -1) run = 0;
+    else if (++run > lmax) lmax = run;
+  }
+  c->Lmax = std::max(2, std::min(lmax, dger::kMaxSentence));
+  if (c->corpus_d.ensure((n_tokens ? n_tokens : 1) * 4)) return fail(DGER_E_ALLOC, "corpus allocation failed");
+  if (n_tokens) DGER_HIP(hipMemcpyAsync(c->corpus_d.p, corpus, n_tokens * 4, hipMemcpyHostToDevice, c->stream));
+  DGER_HIP(hipStreamSynchronize(c->stream));
+  c->n_tokens = n_tokens;
+  c->have_corpus = true;
+  return DGER_OK;
+}
+
+static int auto_workers(dger_ctx* c, size_t lds) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->o.device) != hipSuccess) return 1024;
+  const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                              : (size_t)160 * 1024;
+  int per_cu = (int)std::max<size_t>(1, lds_cu / std::max<size_t>(lds, 1));
+  per_cu = std::min(per_cu, 16);
+  int w = prop.multiProcessorCount * per_cu;
+  // Keep slices long enough that dropped in-flight sentences stay rare.
+  const uint64_t max_by_corpus = std::max<uint64_t>(1, c->n_tokens / 4096);
+  if ((uint64_t)w > max_by_corpus) w = (int)max_by_corpus;
+  return std::max(1, w);
+}
+
+int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, uint64_t* worker_steps,
+                   dger_sgns_stats* stats) {
+  if (!c) return fail(DGER_E_ARG, "null ctx");
+  if (!c->have_corpus) return fail(DGER_E_STATE, "dger_ctx_set_corpus must come first");
+  if (rec && rec_cap <= 0) return fail(DGER_E_ARG, "rec_cap must be positive");
+  DGER_HIP(hipSetDevice(c->o.device));
+  const size_t lds = dger::train_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision);
+  hipDeviceProp_t prop;
+  DGER_HIP(hipGetDeviceProperties(&prop, c->o.device));
+  if (lds > prop.sharedMemPerBlock)
+    return fail(DGER_E_UNSUPPORTED, "sentence length / window / negative need " + std::to_string(lds) +
+                                        " bytes of LDS, more than the device allows");
+  if (workers <= 0) workers = c->o.workers;
+  if (workers <= 0) workers = auto_workers(c, lds);
+  const size_t per = dger::worker_scratch_floats(c->Lmax, c->o.negative, c->D);
+  const int stride = dger::record_stride(c->o.window, c->o.negative);
+  if (c->scratch_d.ensure((size_t)workers * per * 4) || c->wsteps_d.ensure((size_t)workers * 8) ||
+      c->walpha_d.ensure((size_t)workers * 4))
+    return fail(DGER_E_ALLOC, "scratch allocation failed");
+  if (rec && c->rec_d.ensure((size_t)workers * (size_t)rec_cap * stride * 4))
+    return fail(DGER_E_ALLOC, "record allocation failed");
+  DGER_HIP(hipMemsetAsync(c->counters_d.p, 0, dger::kNumCounters * 8, c->stream));
+
+  dger::TrainArgs a;
+  a.corpus = c->corpus_d.as<int32_t>();
+  a.n_tokens = c->n_tokens;
+  a.id_rank = c->id_rank_d.as<int32_t>();
+  a.keep_thr = c->keep_d.as<float>();
+  a.table = c->table_d.as<int32_t>();
+  a.table_size = (uint64_t)c->o.table_size;
+  a.exp_table = c->exp_d.as<float>();
+  a.wih = c->wih_d.p;
+  a.woh = c->woh_d.p;
+  a.fscratch = c->scratch_d.as<float>();
+  a.counters = c->counters_d.as<unsigned long long>();
+  a.rec = rec ? c->rec_d.as<int32_t>() : nullptr;
+  a.rec_cap = rec ? rec_cap : 0;
+  a.worker_steps = c->wsteps_d.as<unsigned long long>();
+  a.worker_alpha = c->walpha_d.as<float>();
+  a.V = (int32_t)c->V;
+  a.D = c->D;
+  a.W = c->o.window;
+  a.K = c->o.negative;
+  a.iter = c->o.iter;
+  a.workers = workers;
+  a.Lmax = c->Lmax;
+  a.sample_on = c->o.sample > 0 ? 1 : 0;
+  a.alpha0 = c->o.alpha;
+  a.alpha_den = (float)((uint64_t)c->o.iter * c->train_words + 1);
+
+  DGER_HIP(hipEventRecord(c->ev0, c->stream));
+  DGER_HIP(dger::launch_train(a, c->o.precision, c->stream));
+  DGER_HIP(hipEventRecord(c->ev1, c->stream));
+  DGER_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  DGER_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+
+  unsigned long long cnt[dger::kNumCounters];
+  DGER_HIP(hipMemcpy(cnt, c->counters_d.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  std::vector<unsigned long long> ws(workers);
+  DGER_HIP(hipMemcpy(ws.data(), c->wsteps_d.p, (size_t)workers * 8, hipMemcpyDeviceToHost));
+  float alpha0 = 0.f;
+  DGER_HIP(hipMemcpy(&alpha0, c->walpha_d.p, 4, hipMemcpyDeviceToHost));
+  if (worker_steps)
+    for (int w = 0; w < workers; w++) worker_steps[w] = ws[w];
+  if (rec)
+    DGER_HIP(hipMemcpy(rec, c->rec_d.p, (size_t)workers * (size_t)rec_cap * stride * 4, hipMemcpyDeviceToHost));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->steps = cnt[dger::kSteps];
+    stats->trained_words = 2 * cnt[dger::kSteps];
+    stats->raw_words = cnt[dger::kRawWords];
+    stats->sentences = cnt[dger::kSentences];
+    stats->final_alpha = alpha0;
+    stats->workers = workers;
+    stats->vocab_size = (int32_t)c->V;
+    stats->max_sentence = c->Lmax;
+    stats->kernel_ms = ms;
+    stats->words_per_sec = ms > 0 ? (double)stats->trained_words / (ms * 1e-3) : 0.0;
+  }
+  return DGER_OK;
+}
+
+int dger_ctx_get_model(dger_ctx* c, float* wih, float* woh) {
+  if (!c) return fail(DGER_E_ARG, "null ctx");
+  DGER_HIP(hipSetDevice(c->o.device));
+  const uint64_t n = (uint64_t)c->V * (uint64_t)c->D;
+  float* outs[2] = {wih, woh};
+  DevBuf* srcs[2] = {&c->wih_d, &c->woh_d};
+  for (int m = 0; m < 2; m++) {
+    if (!outs[m]) continue;
+    if (c->o.precision == DGER_PRECISION_BF16) {
+      DGER_HIP(c->tmp_d.ensure(n * 4));
+      DGER_HIP(dger::launch_bf16_to_f32(srcs[m]->as<uint16_t>(), c->tmp_d.as<float>(), n, c->stream));
+      DGER_HIP(hipMemcpyAsync(outs[m], c->tmp_d.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    } else {
+      DGER_HIP(hipMemcpyAsync(outs[m], srcs[m]->p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    DGER_HIP(hipStreamSynchronize(c->stream));
+  }
+  return DGER_OK;
+}
+
+int dger_ctx_set_model(dger_ctx* c, const float* wih, const float* woh) {
+  if (!c) return fail(DGER_E_ARG, "null ctx");
+  DGER_HIP(hipSetDevice(c->o.device));
+  const uint64_t n = (uint64_t)c->V * (uint64_t)c->D;
+  const float* ins[2] = {wih, woh};
+  DevBuf* dsts[2] = {&c->wih_d, &c->woh_d};
+  for (int m = 0; m < 2; m++) {
+    if (!ins[m]) continue;
+    if (c->o.precision == DGER_PRECISION_BF16) {
+      DGER_HIP(c->tmp_d.ensure(n * 4));
+      DGER_HIP(hipMemcpyAsync(c->tmp_d.p, ins[m], n * 4, hipMemcpyHostToDevice, c->stream));
+      DGER_HIP(dger::launch_f32_to_bf16(c->tmp_d.as<float>(), dsts[m]->as<uint16_t>(), n, c->stream));
+    } else {
+      DGER_HIP(hipMemcpyAsync(dsts[m]->p, ins[m], n * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    DGER_HIP(hipStreamSynchronize(c->stream));
+  }
+  return DGER_OK;
+}
+
+int dger_ctx_get_vocab(dger_ctx* c, int32_t* rank_id, uint64_t* train_words) {
+  if (!c) return fail(DGER_E_ARG, "null ctx");
+  if (rank_id) std::memcpy(rank_id, c->rank_id.data(), c->rank_id.size() * 4);
+  if (train_words) *train_words = c->train_words;
+  return DGER_OK;
+}
+
+int dger_ctx_get_table(dger_ctx* c, int32_t* table) {
+  if (!c || !table) return fail(DGER_E_ARG, "null argument");
+  DGER_HIP(hipSetDevice(c->o.device));
+  DGER_HIP(hipMemcpy(table, c->table_d.p, (size_t)c->o.table_size * 4, hipMemcpyDeviceToHost));
+  return DGER_OK;
+}
+
+// my_saveModel (dsgl.hpp:1443-1461): "V D\n", then per rank the vertex id and
+// D values ("%f " each, or raw floats with -binary), one line per rank.
+int dger_ctx_save(dger_ctx* c, const char* path, int32_t binary) {
+  if (!c || !path) return fail(DGER_E_ARG, "null argument");
+  std::vector<float> wih((size_t)c->V * c->D);
+  int rc = dger_ctx_get_model(c, wih.data(), nullptr);
+  if (rc) return rc;
+  FILE* fo = std::fopen(path, "wb");
+  if (!fo) return fail(DGER_E_IO, std::string("cannot open ") + path);
+  std::fprintf(fo, "%ld %ld\n", (long)c->V, (long)c->D);
+  for (uint32_t a = 0; a < c->V; a++) {
+    std::fprintf(fo, "%u ", (unsigned)c->rank_id[a]);
+    const float* row = &wih[(size_t)a * c->D];
+    if (binary) {
+      std::fwrite(row, sizeof(float), (size_t)c->D, fo);
+    } else {
+      for (int b = 0; b < c->D; b++) std::fprintf(fo, "%f ", row[b]);
+    }
+    std::fprintf(fo, "\n");
+  }
+  if (std::fclose(fo) != 0) return fail(DGER_E_IO, std::string("write failed: ") + path);
+  return DGER_OK;
+}
+
+void dger_ctx_destroy(dger_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->o.device);
+  delete c;
+}
+
+int dger_sgns_train(const dger_sgns_opts* opts, const int32_t* corpus, uint64_t n_tokens,
+                    const int32_t* vertex_cn, uint32_t v_num, float* emb_out, int32_t* rank_id_out,
+                    dger_sgns_stats* stats) {
+  dger_ctx* c = nullptr;
+  int rc = dger_ctx_create(opts, vertex_cn, v_num, &c);
+  if (rc) return rc;
+  rc = dger_ctx_set_corpus(c, corpus, n_tokens);
+  if (!rc) rc = dger_ctx_train(c, 0, nullptr, 0, nullptr, stats);
+  if (!rc && emb_out) rc = dger_ctx_get_model(c, emb_out, nullptr);
+  if (!rc && rank_id_out) rc = dger_ctx_get_vocab(c, rank_id_out, nullptr);
+  dger_ctx_destroy(c);
+  return rc;
+}
+
+int dger_sgns_main(int argc, char** argv, const int32_t* corpus, uint64_t n_tokens,
+                   const int32_t* vertex_cn, uint32_t v_num) {
+  dger_sgns_opts o;
+  int rc = dger_sgns_parse_args(argc, argv, &o);
+  if (rc) return rc;
+  if (v_num <= 1000) return fail(DGER_E_VOCAB, "dsgl() needs more than 1000 vertices (dsgl.hpp:370)");
+  dger_ctx* c = nullptr;
+  rc = dger_ctx_create(&o, vertex_cn, v_num, &c);
+  if (rc) return rc;
+  dger_sgns_stats st;
+  rc = dger_ctx_set_corpus(c, corpus, n_tokens);
+  if (!rc) rc = dger_ctx_train(c, 0, nullptr, 0, nullptr, &st);
+  if (!rc && o.debug > 0)
+    std::printf("> [Train SGNS TIME:] %lf  trained words/s: %.1f  workers: %d\n", st.kernel_ms * 1e-3,
+                st.words_per_sec, st.workers);
+  if (!rc && o.eoutput[0]) rc = dger_ctx_save(c, o.eoutput, o.binary);
+  dger_ctx_destroy(c);
+  return rc;
+}
+
+// ---- test-facing kernels ------------------------------------------------------
+int dger_sgns_step(int32_t D, int32_t n_out, int32_t n_ctx, int32_t n_ctx_first, const int32_t* meta,
+                   float alpha, const float* O, const float* I, float* corr, float* md, float* gin) {
+  if (D < 2 || D > 1024 || (D & 1) || n_out < 2 || n_ctx < 1 || n_ctx_first < 0 || n_ctx_first > n_ctx)
+    return fail(DGER_E_ARG, "bad step shape");
+  float exp_table[dger::kExpTable];
+  build_exp_table(exp_table);
+  DevBuf dm, de, dO, dI, dc, dmd, dg;
+  if (dm.ensure(n_out * 4) || de.ensure(sizeof(exp_table)) || dO.ensure((size_t)n_out * D * 4) ||
+      dI.ensure((size_t)n_ctx * D * 4) || dc.ensure((size_t)n_out * n_ctx * 4) ||
+      dmd.ensure((size_t)n_out * D * 4) || dg.ensure((size_t)n_ctx * D * 4))
+    return fail(DGER_E_ALLOC, "device allocation failed");
+  DGER_HIP(hipMemcpy(dm.p, meta, n_out * 4, hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(de.p, exp_table, sizeof(exp_table), hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(dO.p, O, (size_t)n_out * D * 4, hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(dI.p, I, (size_t)n_ctx * D * 4, hipMemcpyHostToDevice));
+  DGER_HIP(dger::launch_step(D, n_out, n_ctx, n_ctx_first, dm.as<int32_t>(), alpha, de.as<float>(),
+                             dO.as<float>(), dI.as<float>(), dc.as<float>(), dmd.as<float>(), dg.as<float>(),
+                             nullptr));
+  DGER_HIP(hipDeviceSynchronize());
+  DGER_HIP(hipMemcpy(corr, dc.p, (size_t)n_out * n_ctx * 4, hipMemcpyDeviceToHost));
+  DGER_HIP(hipMemcpy(md, dmd.p, (size_t)n_out * D * 4, hipMemcpyDeviceToHost));
+  DGER_HIP(hipMemcpy(gin, dg.p, (size_t)n_ctx * D * 4, hipMemcpyDeviceToHost));
+  return DGER_OK;
+}
+
+#define DGER_LOSS_CHECK()                                                                   \
+  if (!syn0 || !syn1 || !ctx || !out || D < 1 || n_mb < 1 || n_ctx < 1 || n_out < 1)        \
+    return fail(DGER_E_ARG, "bad loss arguments");
+
+int dger_loss_fwd_f32(const float* syn0, const float* syn1, int32_t D, const int32_t* ctx, const int32_t* out,
+                      int32_t n_mb, int32_t n_ctx, int32_t n_out, float* loss_out) {
+  DGER_LOSS_CHECK()
+  DGER_HIP(dger::launch_loss_fwd<float>(syn0, syn1, D, ctx, out, n_mb, n_ctx, n_out, loss_out, nullptr));
+  DGER_HIP(hipDeviceSynchronize());
+  return DGER_OK;
+}
+
+int dger_loss_bwd_f32(const float* syn0, const float* syn1, int32_t D, const int32_t* ctx, const int32_t* out,
+                      int32_t n_mb, int32_t n_ctx, int32_t n_out, float scale, float* g0, float* g1) {
+  DGER_LOSS_CHECK()
+  DGER_HIP(dger::launch_loss_bwd<float>(syn0, syn1, D, ctx, out, n_mb, n_ctx, n_out, scale, g0, g1, nullptr));
+  DGER_HIP(hipDeviceSynchronize());
+  return DGER_OK;
+}
+
+int dger_loss_fwd_f64(const double* syn0, const double* syn1, int32_t D, const int32_t* ctx,
+                      const int32_t* out, int32_t n_mb, int32_t n_ctx, int32_t n_out, double* loss_out) {
+  DGER_LOSS_CHECK()
+  DGER_HIP(dger::launch_loss_fwd<double>(syn0, syn1, D, ctx, out, n_mb, n_ctx, n_out, loss_out, nullptr));
+  DGER_HIP(hipDeviceSynchronize());
+  return DGER_OK;
+}
+
+int dger_loss_bwd_f64(const double* syn0, const double* syn1, int32_t D, const int32_t* ctx,
+                      const int32_t* out, int32_t n_mb, int32_t n_ctx, int32_t n_out, double scale, double* g0,
+                      double* g1) {
+  DGER_LOSS_CHECK()
+  DGER_HIP(dger::launch_loss_bwd<double>(syn0, syn1, D, ctx, out, n_mb, n_ctx, n_out, scale, g0, g1, nullptr));
+  DGER_HIP(hipDeviceSynchronize());
+  return DGER_OK;
+}
+
+int dger_dev_alloc(void** ptr, uint64_t bytes) {
+  if (!ptr) return fail(DGER_E_ARG, "null pointer");
+  DGER_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return DGER_OK;
+}
+int dger_dev_free(void* ptr) {
+  DGER_HIP(hipFree(ptr));
+  return DGER_OK;
+}
+int dger_dev_upload(void* dst, const void* src, uint64_t bytes) {
+  DGER_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return DGER_OK;
+}
+int dger_dev_download(void* dst, const void* src, uint64_t bytes) {
+  DGER_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return DGER_OK;
+}
+int dger_dev_zero(void* ptr, uint64_t bytes) {
+  DGER_HIP(hipMemset(ptr, 0, bytes));
+  return DGER_OK;
+}
+int dger_device_count(int32_t* count) {
+  if (!count) return fail(DGER_E_ARG, "null pointer");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return DGER_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// Internal interface between the host layer (sgns_host.cpp) and the gfx950
+// kernels (sgns_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+namespace dger {
+
+constexpr unsigned long long kLcgMul = 25214903917ULL;  // dsgl.hpp:453
+constexpr unsigned long long kLcgAdd = 11ULL;
+constexpr int kMaxSentence = 1000;                     // dsgl.hpp:23
+constexpr int kExpTable = 1001;                        // dsgl.hpp:20, :1557
+
+// Counters kept in device memory for one training run.
+enum Counter : int {
+  kWordCountActual = 0,  // word_count_actual (dsgl.hpp:535), shared by all workers
+  kSteps = 1,
+  kRawWords = 2,
+  kSentences = 3,
+  kNumCounters = 4,
+};
+
+struct TrainArgs {
+  const int32_t* corpus;     // n_tokens vertex ids, -1 after each walk
+  uint64_t n_tokens;
+  const int32_t* id_rank;    // V: vertex id -> vocabulary rank
+  const float* keep_thr;     // V: subsampling threshold per rank
+  const int32_t* table;      // noise table over ranks
+  uint64_t table_size;
+  const float* exp_table;    // 1001-entry sigmoid table
+  void* wih;                 // V x D (float or bf16 rows)
+  void* woh;                 // V x D
+  float* fscratch;           // per worker: 2 sentence buffers + negative buffer
+  unsigned long long* counters;
+  int32_t* rec;              // optional index-stream records (device)
+  int64_t rec_cap;
+  unsigned long long* worker_steps;  // per worker
+  float* worker_alpha;       // per worker, final learning rate
+  int32_t V, D, W, K, iter, workers, Lmax;
+  int32_t sample_on;
+  float alpha0;
+  float alpha_den;           // (float)(iter * train_words + 1)
+};
+
+// Floats of per-worker scratch: two sentence buffers (Lmax rows each) and the
+// negative buffer (K * Lmax rows).
+#if defined(__HIPCC__)
+#define DGER_HD __host__ __device__
+#else
+#define DGER_HD
+#endif
+
+DGER_HD inline size_t worker_scratch_floats(int Lmax, int K, int D) {
+  return (size_t)(2 + K) * (size_t)Lmax * (size_t)D;
+}
+
+size_t train_lds_bytes(int Lmax, int K, int W, int D, int precision);
+DGER_HD inline int record_stride(int W, int K) { return 8 + 4 * W + 2 * (K + 2); }
+
+hipError_t launch_init_net(float* wih, uint64_t n, int D, hipStream_t s);
+hipError_t launch_table_fill(int32_t* table, uint64_t table_size, const int64_t* starts, int n_starts,
+                             hipStream_t s);
+hipError_t launch_train(const TrainArgs& a, int precision, hipStream_t s);
+hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s);
+hipError_t launch_bf16_to_f32(const uint16_t* src, float* dst, uint64_t n, hipStream_t s);
+hipError_t launch_step(int D, int os, int nin, int fin, const int32_t* meta, float alpha,
+                       const float* exp_table, const float* O, const float* I, float* corr,
+                       float* md, float* gin, hipStream_t s);
+template <typename T>
+hipError_t launch_loss_fwd(const T* syn0, const T* syn1, int D, const int32_t* ctx,
+                           const int32_t* out, int n_mb, int n_ctx, int n_out, T* loss,
+                           hipStream_t s);
+template <typename T>
+hipError_t launch_loss_bwd(const T* syn0, const T* syn1, int D, const int32_t* ctx,
+                           const int32_t* out, int n_mb, int n_ctx, int n_out, T scale, T* g0,
+                           T* g1, hipStream_t s);
+
+}  // namespace dger

@@ -1,0 +1,364 @@
+"""Python front end of libdger_sgns.so (include/dger_sgns.h) over ctypes.
+
+The library is the product: a gfx950 HIP implementation of DistGER's DSGL
+skip-gram negative-sampling trainer (reference: include/dsgl.hpp).  This module
+only marshals arguments; there is no Python or CPU fallback.  If the shared
+library is missing the import of any entry point raises ``LibraryMissing``.
+
+The function ``dsgl`` mirrors the reference entry point
+``int dsgl(int argc, char **argv, vector<int> *vertex_cn, ...)``
+(include/dsgl.hpp:1464): argv-style options with the same flag names, the
+corpus as vertex ids with -1 after every walk, and per-vertex counts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdger_sgns.so")
+
+DGER_OK = 0
+DGER_E_ARG = -1
+DGER_E_CORPUS = -2
+DGER_E_VOCAB = -3
+DGER_E_HIP = -4
+DGER_E_ALLOC = -5
+DGER_E_IO = -6
+DGER_E_STATE = -7
+DGER_E_UNSUPPORTED = -8
+PRECISION = {"fp32": 0, "bf16": 1}
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class DgerError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dger error {code}: {msg}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("size", C.c_int32), ("window", C.c_int32), ("negative", C.c_int32),
+        ("batch_size", C.c_int32), ("sample", C.c_float), ("alpha", C.c_float),
+        ("min_count", C.c_int32), ("iter", C.c_int32), ("threads", C.c_int32),
+        ("debug", C.c_int32), ("binary", C.c_int32), ("disk", C.c_int32),
+        ("sync_period", C.c_float), ("min_sync_words", C.c_int32),
+        ("full_sync_times", C.c_int32), ("message_size", C.c_int32),
+        ("eoutput", C.c_char * 512), ("save_vocab", C.c_char * 512),
+        ("read_vocab", C.c_char * 512), ("save_emb", C.c_char * 512),
+        ("workers", C.c_int32), ("precision", C.c_int32), ("table_size", C.c_int64),
+        ("device", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_uint64), ("trained_words", C.c_uint64), ("raw_words", C.c_uint64),
+        ("sentences", C.c_uint64), ("final_alpha", C.c_float), ("workers", C.c_int32),
+        ("vocab_size", C.c_int32), ("max_sentence", C.c_int32), ("kernel_ms", C.c_double),
+        ("words_per_sec", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_vp = C.c_void_p
+_i32 = C.c_int32
+_SIGS = {
+    "dger_last_error": (C.c_char_p, []),
+    "dger_record_stride": (_i32, [_i32, _i32]),
+    "dger_sgns_default_opts": (C.c_int, [C.POINTER(Opts)]),
+    "dger_sgns_parse_args": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(Opts)]),
+    "dger_sgns_train": (C.c_int, [C.POINTER(Opts), _vp, C.c_uint64, _vp, C.c_uint32, _vp, _vp,
+                                  C.POINTER(Stats)]),
+    "dger_sgns_main": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), _vp, C.c_uint64, _vp, C.c_uint32]),
+    "dger_ctx_create": (C.c_int, [C.POINTER(Opts), _vp, C.c_uint32, C.POINTER(_vp)]),
+    "dger_ctx_set_corpus": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "dger_ctx_reset_model": (C.c_int, [_vp]),
+    "dger_ctx_train": (C.c_int, [_vp, _i32, _vp, C.c_int64, _vp, C.POINTER(Stats)]),
+    "dger_ctx_get_model": (C.c_int, [_vp, _vp, _vp]),
+    "dger_ctx_set_model": (C.c_int, [_vp, _vp, _vp]),
+    "dger_ctx_get_vocab": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
+    "dger_ctx_get_table": (C.c_int, [_vp, _vp]),
+    "dger_ctx_save": (C.c_int, [_vp, C.c_char_p, _i32]),
+    "dger_ctx_destroy": (None, [_vp]),
+    "dger_sgns_step": (C.c_int, [_i32, _i32, _i32, _i32, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp]),
+    "dger_loss_fwd_f32": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp]),
+    "dger_loss_bwd_f32": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, C.c_float, _vp, _vp]),
+    "dger_loss_fwd_f64": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp]),
+    "dger_loss_bwd_f64": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, C.c_double, _vp, _vp]),
+    "dger_dev_alloc": (C.c_int, [C.POINTER(_vp), C.c_uint64]),
+    "dger_dev_free": (C.c_int, [_vp]),
+    "dger_dev_upload": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "dger_dev_download": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "dger_dev_zero": (C.c_int, [_vp, C.c_uint64]),
+    "dger_device_count": (C.c_int, [C.POINTER(_i32)]),
+}
+EXPORTS = tuple(_SIGS)
+
+_lib = None
+
+
+def lib():
+    """Load libdger_sgns.so (raises LibraryMissing if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != DGER_OK:
+        raise DgerError(rc, lib().dger_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def device_count() -> int:
+    n = _i32(0)
+    _check(lib().dger_device_count(C.byref(n)))
+    return int(n.value)
+
+
+def record_stride(window: int, negative: int) -> int:
+    return int(lib().dger_record_stride(window, negative))
+
+
+def _argv(args):
+    arr = (C.c_char_p * (len(args) + 1))()
+    for i, a in enumerate(args):
+        arr[i] = a.encode() if isinstance(a, str) else a
+    arr[len(args)] = None
+    return arr
+
+
+def parse_args(argv) -> Opts:
+    """dger_sgns_parse_args over a list of strings (argv[0] is the program)."""
+    o = Opts()
+    _check(lib().dger_sgns_parse_args(len(argv), _argv(argv), C.byref(o)))
+    return o
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    _check(lib().dger_sgns_default_opts(C.byref(o)))
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = PRECISION[v]
+        if k == "eoutput":
+            v = v.encode()
+        setattr(o, k, v)
+    return o
+
+
+def _i32arr(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Trainer:
+    """Device-resident trainer (the dger_ctx_* API)."""
+
+    def __init__(self, opts: Opts, vertex_cn, corpus=None):
+        self.opts = opts
+        self.cn = _i32arr(vertex_cn)
+        self.V = int(self.cn.size)
+        self.D = int(opts.size)
+        h = _vp()
+        _check(lib().dger_ctx_create(C.byref(opts), _ptr(self.cn), self.V, C.byref(h)))
+        self._h = h
+        if corpus is not None:
+            self.set_corpus(corpus)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().dger_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_corpus(self, corpus) -> None:
+        c = _i32arr(corpus)
+        _check(lib().dger_ctx_set_corpus(self._h, _ptr(c), c.size))
+
+    def reset_model(self) -> None:
+        _check(lib().dger_ctx_reset_model(self._h))
+
+    def train(self, workers: int = 0, rec_cap: int = 0):
+        """Returns (stats, records [workers, rec_cap, stride] or None, worker_steps)."""
+        st = Stats()
+        nw = workers if workers > 0 else (self.opts.workers or 0)
+        rec = None
+        ws = None
+        if nw > 0:
+            ws = np.zeros(nw, np.uint64)
+            if rec_cap > 0:
+                rec = np.full((nw, rec_cap, record_stride(self.opts.window, self.opts.negative)), -1,
+                              np.int32)
+        elif rec_cap > 0:
+            raise ValueError("records need an explicit worker count")
+        _check(lib().dger_ctx_train(self._h, workers, _ptr(rec), rec_cap, _ptr(ws), C.byref(st)))
+        return st, rec, ws
+
+    def get_model(self):
+        wih = np.empty((self.V, self.D), np.float32)
+        woh = np.empty((self.V, self.D), np.float32)
+        _check(lib().dger_ctx_get_model(self._h, _ptr(wih), _ptr(woh)))
+        return wih, woh
+
+    def set_model(self, wih=None, woh=None) -> None:
+        a = None if wih is None else np.ascontiguousarray(wih, np.float32)
+        b = None if woh is None else np.ascontiguousarray(woh, np.float32)
+        _check(lib().dger_ctx_set_model(self._h, _ptr(a), _ptr(b)))
+
+    def vocab(self):
+        rank_id = np.empty(self.V, np.int32)
+        tw = C.c_uint64(0)
+        _check(lib().dger_ctx_get_vocab(self._h, _ptr(rank_id), C.byref(tw)))
+        return rank_id, int(tw.value)
+
+    def table(self) -> np.ndarray:
+        t = np.empty(int(self.opts.table_size), np.int32)
+        _check(lib().dger_ctx_get_table(self._h, _ptr(t)))
+        return t
+
+    def save(self, path: str, binary: int = 0) -> None:
+        _check(lib().dger_ctx_save(self._h, path.encode(), binary))
+
+
+def train(opts: Opts, corpus, vertex_cn):
+    """dger_sgns_train: returns (embeddings V x D in rank order, rank_id, stats)."""
+    c = _i32arr(corpus)
+    cn = _i32arr(vertex_cn)
+    emb = np.empty((cn.size, opts.size), np.float32)
+    rank_id = np.empty(cn.size, np.int32)
+    st = Stats()
+    _check(lib().dger_sgns_train(C.byref(opts), _ptr(c), c.size, _ptr(cn), cn.size, _ptr(emb),
+                                 _ptr(rank_id), C.byref(st)))
+    return emb, rank_id, st
+
+
+def dsgl(argv, vertex_cn, local_corpus) -> int:
+    """Mirror of dsgl(argc, argv, &vertex_cn, &new_sort, &graph) (include/dsgl.hpp:1464).
+
+    Returns 0 on success; raises DgerError with the library's code otherwise
+    (the reference exits or asserts instead)."""
+    c = _i32arr(local_corpus)
+    cn = _i32arr(vertex_cn)
+    _check(lib().dger_sgns_main(len(argv), _argv(argv), _ptr(c), c.size, _ptr(cn), cn.size))
+    return 0
+
+
+def step(O, I, fin: int, meta, alpha: float):
+    """One minibatch step on the device (dger_sgns_step): returns corr, md, gin."""
+    O = np.ascontiguousarray(O, np.float32)
+    I = np.ascontiguousarray(I, np.float32)
+    meta = _i32arr(meta)
+    os_, D = O.shape
+    nin = I.shape[0]
+    corr = np.empty((os_, nin), np.float32)
+    md = np.empty((os_, D), np.float32)
+    gin = np.empty((nin, D), np.float32)
+    _check(lib().dger_sgns_step(D, os_, nin, fin, _ptr(meta), alpha, _ptr(O), _ptr(I), _ptr(corr),
+                                _ptr(md), _ptr(gin)))
+    return corr, md, gin
+
+
+class DeviceArray:
+    """A device allocation made through the library (for the loss kernels)."""
+
+    def __init__(self, host: np.ndarray):
+        self.dtype = host.dtype
+        self.shape = host.shape
+        self.nbytes = int(host.nbytes)
+        p = _vp()
+        _check(lib().dger_dev_alloc(C.byref(p), self.nbytes))
+        self.ptr = p
+        host = np.ascontiguousarray(host)
+        _check(lib().dger_dev_upload(self.ptr, _ptr(host), self.nbytes))
+
+    @classmethod
+    def zeros(cls, shape, dtype):
+        return cls(np.zeros(shape, dtype))
+
+    def zero(self):
+        _check(lib().dger_dev_zero(self.ptr, self.nbytes))
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().dger_dev_download(_ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().dger_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def loss_fwd(syn0: DeviceArray, syn1: DeviceArray, ctx: DeviceArray, out: DeviceArray,
+             n_mb: int, n_ctx: int, n_out: int) -> float:
+    f64 = syn0.dtype == np.float64
+    acc = DeviceArray.zeros((1,), np.float64 if f64 else np.float32)
+    fn = lib().dger_loss_fwd_f64 if f64 else lib().dger_loss_fwd_f32
+    _check(fn(syn0.ptr, syn1.ptr, syn0.shape[1], ctx.ptr, out.ptr, n_mb, n_ctx, n_out, acc.ptr))
+    v = float(acc.get()[0])
+    acc.free()
+    return v
+
+
+def loss_bwd(syn0: DeviceArray, syn1: DeviceArray, ctx: DeviceArray, out: DeviceArray,
+             n_mb: int, n_ctx: int, n_out: int, scale: float = 1.0):
+    f64 = syn0.dtype == np.float64
+    g0 = DeviceArray.zeros(syn0.shape, syn0.dtype)
+    g1 = DeviceArray.zeros(syn1.shape, syn1.dtype)
+    fn = lib().dger_loss_bwd_f64 if f64 else lib().dger_loss_bwd_f32
+    _check(fn(syn0.ptr, syn1.ptr, syn0.shape[1], ctx.ptr, out.ptr, n_mb, n_ctx, n_out, scale,
+              g0.ptr, g1.ptr))
+    r = (g0.get(), g1.get())
+    g0.free()
+    g1.free()
+    return r
+
+
+@dataclass
+class Corpus:
+    """A walk corpus in the reference's in-memory layout (include/path.hpp:91-96,110)."""
+    tokens: np.ndarray      # int32 vertex ids, -1 after every walk
+    vertex_cn: np.ndarray   # int32 occurrences per vertex id
+    n_walks: int
+
+    @property
+    def n_tokens(self) -> int:
+        return int(self.tokens.size)
