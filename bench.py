@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X SGNS trainer (DistGER DSGL hot path) on one GPU.
+
+A "step" is one training pass (iter = 1) of the trainer over the whole
+synthetic corpus, with the corpus, model and tables already resident in HBM.
+The workload is the R-MAT 2^21-vertex shard (average degree 16, (a,b,c) =
+(0.57, 0.19, 0.19), seed 3, 1 % of the edges held out for link prediction)
+with about 1e8 walk tokens of log-normal(3.7, 0.5) lengths clipped to
+[5, 120]; dim 128, window 10, negative 5, sample 1e-3, alpha 0.01.
+
+The printed JSON line reports trained words/s (post-subsampling target
+positions) of the BF16-row form as ``value``, the FP32 form beside it, the
+link-prediction AUC of each form after one pass, the roofline of the trainer
+kernel from HIP-event kernel time and algorithmic bytes, and the CPU baseline
+(the oracle's OpenMP Hogwild trainer on the host cores, on a bounded prefix
+sample of the same corpus).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gdistger-re_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_word(D: int, K: int, row_bytes: int, S: float = 2.0) -> float:
+    """Compulsory HBM bytes per trained word: token id, the word's input row
+    read at sentence load and written back at sentence end, its target row
+    read and written, and K negative rows read and written shared by the S=2
+    targets of a step: 4 + 2R + 2R(1 + K/S)."""
+    R = D * row_bytes
+    return 4 + 2 * R + 2 * R * (1 + K / S)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=21)
+    ap.add_argument("--tokens", type=float, default=1e8)
+    ap.add_argument("--workers", type=int, default=0, help="0 = library default")
+    ap.add_argument("--forms", default="fp32,bf16")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-auc", action="store_true")
+    ap.add_argument("--cpu-tokens", type=float, default=16e6,
+                    help="prefix of the corpus the CPU baseline trains on")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import dger_sgns as G
+    from dger_sgns import synth
+
+    D, W, K, sample, alpha = 128, 10, 5, 1e-3, 0.01
+    t0 = time.time()
+    corpus, test_pairs, test_labels = synth.rmat_corpus(
+        scale=args.scale, avg_degree=16, abc=(0.57, 0.19, 0.19), seed=3,
+        target_tokens=int(args.tokens), holdout=0.0 if args.no_auc else 0.01)
+    log(f"corpus: {corpus.n_tokens} tokens, {corpus.n_walks} walks, V={corpus.vertex_cn.size} "
+        f"({time.time() - t0:.1f}s)")
+
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    results = {}
+    for form in forms:
+        opts = G.default_opts(size=D, window=W, negative=K, sample=sample, alpha=alpha, iter=1,
+                              threads=2, workers=args.workers, precision=form, debug=0)
+        t1 = time.time()
+        tr = G.Trainer(opts, corpus.vertex_cn, corpus.tokens)
+        log(f"[{form}] prepared in {time.time() - t1:.1f}s")
+        for i in range(args.warmup):
+            st, _, _ = tr.train()
+            log(f"[{form}] warmup {i}: {st.trained_words} words, {st.kernel_ms:.1f} ms kernel")
+        words = 0
+        kms = []
+        wall0 = time.perf_counter()
+        for i in range(args.steps):
+            st, _, _ = tr.train()
+            words += st.trained_words
+            kms.append(st.kernel_ms)
+            log(f"[{form}] step {i}: {st.trained_words} words, {st.kernel_ms:.1f} ms kernel, "
+                f"{st.words_per_sec / 1e6:.1f} M words/s, workers {st.workers}")
+        wall = time.perf_counter() - wall0
+        res = {
+            "words_per_sec": words / wall,
+            "ms_per_step": 1e3 * wall / max(1, args.steps),
+            "kernel_ms_avg": float(np.mean(kms)) if kms else 0.0,
+            "trained_words_per_step": int(st.trained_words),
+            "raw_words_per_step": int(st.raw_words),
+            "workers": int(st.workers),
+            "max_sentence": int(st.max_sentence),
+        }
+        if not args.no_auc:
+            tr.reset_model()
+            tr.train()
+            rank_id, _ = tr.vocab()
+            wih, _ = tr.get_model()
+            res["auc"] = synth.link_auc(wih, rank_id, test_pairs, test_labels)
+            log(f"[{form}] link-prediction AUC after one pass: {res['auc']:.4f}")
+        tr.close()
+        results[form] = res
+
+    head = "bf16" if "bf16" in results else forms[0]
+    h = results[head]
+    row_bytes = 2 if head == "bf16" else 4
+    bpw = algorithmic_bytes_per_word(D, K, row_bytes)
+    achieved = h["trained_words_per_step"] * bpw / (h["kernel_ms_avg"] * 1e-3) / 1e9
+    out = {
+        "metric": "trained words/sec, BF16 rows vs FP32 plain, at <=0.005 link-pred AUC gap",
+        "value": h["words_per_sec"],
+        "unit": "words/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if head == "bf16" else "fp32",
+        "data": "synthetic (R-MAT graph, uniform random walks; seeded)",
+        "config": {
+            "workload": f"rmat-s{args.scale}-deg16-lognormal3.7-{corpus.n_tokens}tok",
+            "dim": D, "window": W, "negative": K, "sample": sample, "alpha": alpha, "iter": 1,
+            "tokens": corpus.n_tokens, "walks": corpus.n_walks, "vocab": int(corpus.vertex_cn.size),
+            "workers": h["workers"], "form": head,
+        },
+        "forms": results,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_train",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_word": bpw,
+        },
+    }
+    if "fp32" in results and "bf16" in results:
+        out["bf16_over_fp32"] = results["bf16"]["words_per_sec"] / results["fp32"]["words_per_sec"]
+        if "auc" in results["fp32"]:
+            out["auc_gap"] = abs(results["bf16"]["auc"] - results["fp32"]["auc"])
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(corpus, D, W, K, sample, alpha, int(args.cpu_tokens))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(corpus, D, W, K, sample, alpha, n_tokens):
+    """The oracle's OpenMP Hogwild trainer (one worker per host core) on a
+    prefix of the corpus; trained words/s over the training loop only."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as orc
+    cores = min(os.cpu_count() or 1, 16)
+    prefix = corpus.tokens[:n_tokens]
+    cut = np.nonzero(prefix == -1)[0]
+    prefix = prefix[:cut[-1] + 1] if cut.size else prefix
+    opts = orc.make_opts(dim=D, window=W, negative=K, iter=1, threads=cores + 1, workers=cores,
+                         sample=sample, alpha=alpha, parallel=1)
+    log(f"cpu baseline: {prefix.size} tokens on {cores} threads")
+    _, _, st, _, _ = orc.train(prefix, corpus.vertex_cn, opts, fast=True)
+    return {
+        "value": st.trained_words / st.train_seconds,
+        "unit": "words/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"first {prefix.size} tokens of the same corpus, same vocabulary and options, "
+                  f"{cores} OpenMP Hogwild workers, {st.train_seconds:.1f}s of training",
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

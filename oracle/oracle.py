@@ -59,6 +59,7 @@ class OrcStats(C.Structure):
     _fields_ = [
         ("steps", C.c_uint64), ("trained_words", C.c_uint64), ("raw_words", C.c_uint64),
         ("sentences", C.c_uint64), ("final_alpha", C.c_float), ("vocab_size", C.c_int32),
+        ("train_seconds", C.c_double),
     ]
 
 

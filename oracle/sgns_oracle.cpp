@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -253,6 +254,7 @@ typedef struct orc_stats {
   uint64_t sentences;     // sentences trained to their end
   float final_alpha;
   int32_t vocab_size;
+  double train_seconds;   // wall time of the worker loop only (as dsgl.hpp:1571-1573)
 } orc_stats;
 
 int32_t orc_record_stride(int32_t window, int32_t negative) {
@@ -556,6 +558,7 @@ int orc_train(const orc_opts* o, const int32_t* corpus, uint64_t n_tokens, const
   sh.woh = woh;
   const int stride = orc_record_stride(o->window, o->negative);
   std::vector<WorkerResult> res(o->workers);
+  const auto t0 = std::chrono::steady_clock::now();
   if (o->parallel) {
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static, 1) num_threads(o->workers)
@@ -566,8 +569,10 @@ int orc_train(const orc_opts* o, const int32_t* corpus, uint64_t n_tokens, const
     for (int w = 0; w < o->workers; w++)
       res[w] = run_worker(sh, w, rec ? rec + (size_t)w * rec_cap * stride : nullptr, rec_cap);
   }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
+    stats->train_seconds = secs;
     for (int w = 0; w < o->workers; w++) {
       stats->steps += res[w].steps;
       stats->raw_words += res[w].raw;

@@ -1,0 +1,121 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/dger_sgns.h declares, parses options like dsgl() (dsgl.hpp:1496-1535),
+and rejects bad inputs with error codes before touching the device."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import dger_sgns as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dger_sgns.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(dger_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_function():
+    lib = G.lib()
+    declared = header_functions()
+    assert len(declared) >= 25
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(G.EXPORTS)
+
+
+def test_default_options_are_the_reference_defaults():
+    o = G.default_opts()
+    # dsgl.hpp:67-76
+    assert (o.size, o.window, o.negative, o.batch_size, o.min_count, o.iter, o.threads) == \
+        (100, 5, 5, 11, 5, 5, 12)
+    assert np.float32(o.alpha) == np.float32(0.1) and np.float32(o.sample) == np.float32(1e-3)
+    assert o.table_size == 100_000_000 and o.precision == 0 and o.workers == 0
+
+
+def test_parse_args_flags():
+    o = G.parse_args(["prog", "-size", "128", "-window", "10", "-negative", "5", "-batch-size", "21",
+                      "-sample", "1e-3", "-alpha", "0.01", "-min-count", "0", "-iter", "1",
+                      "-eoutput", "emb.txt", "-threads", "9", "-unknown-flag", "3", "-disk",
+                      "-binary", "1", "-precision", "bf16", "-workers", "64"])
+    assert (o.size, o.window, o.negative, o.batch_size, o.min_count, o.iter, o.threads) == \
+        (128, 10, 5, 21, 0, 1, 9)
+    assert o.eoutput == b"emb.txt" and o.disk == 1 and o.binary == 1
+    assert o.precision == 1 and o.workers == 64
+    assert np.float32(o.alpha) == np.float32(0.01)
+
+
+def test_parse_args_missing_value():
+    with pytest.raises(G.DgerError) as e:
+        G.parse_args(["prog", "-window"])
+    assert e.value.code == G.DGER_E_ARG
+    with pytest.raises(G.DgerError) as e:
+        G.parse_args(["prog", "-precision", "fp16"])
+    assert e.value.code == G.DGER_E_ARG
+
+
+def test_bad_options_rejected_before_device():
+    cn = np.ones(10, np.int32)
+    for kw, code in [(dict(size=7), G.DGER_E_UNSUPPORTED), (dict(size=2048), G.DGER_E_UNSUPPORTED),
+                     (dict(window=0), G.DGER_E_UNSUPPORTED), (dict(negative=-1), G.DGER_E_UNSUPPORTED),
+                     (dict(iter=0), G.DGER_E_ARG), (dict(table_size=0), G.DGER_E_ARG)]:
+        with pytest.raises(G.DgerError) as e:
+            G.Trainer(G.default_opts(**kw), cn)
+        assert e.value.code == code, kw
+    with pytest.raises(G.DgerError) as e:
+        G.Trainer(G.default_opts(size=8), np.ones(1, np.int32))
+    assert e.value.code == G.DGER_E_VOCAB
+    with pytest.raises(G.DgerError) as e:
+        G.Trainer(G.default_opts(size=8), np.array([1, -1, 2], np.int32))
+    assert e.value.code == G.DGER_E_ARG
+
+
+def test_dsgl_small_vocabulary_rejected():
+    # dsgl.hpp:370 asserts a vocabulary of more than 1000 vertices.
+    cn = np.ones(1000, np.int32)
+    corpus = np.array([1, 2, 3, -1], np.int32)
+    with pytest.raises(G.DgerError) as e:
+        G.dsgl(["prog", "-size", "8"], cn, corpus)
+    assert e.value.code == G.DGER_E_VOCAB
+
+
+def test_no_device_reports_hip_error():
+    if G.device_count() > 0:
+        pytest.skip("a HIP device is present; this checks the no-device error path")
+    with pytest.raises(G.DgerError) as e:
+        G.Trainer(G.default_opts(size=8), np.ones(10, np.int32))
+    assert e.value.code in (G.DGER_E_HIP, G.DGER_E_ARG)
+
+
+def test_record_stride():
+    assert G.record_stride(10, 5) == 8 + 40 + 14
+
+
+def test_cxx_shim_links(tmp_path):
+    """A C++ caller of the header-only dsgl() shim compiles, links and gets the
+    vocabulary error code back without a device."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(r'''
+#include "dger_sgns.h"
+#include <cstdio>
+int main() {
+  std::vector<int> cn(10, 1), corpus = {1, 2, 3, -1};
+  std::vector<uint32_t> new_sort;
+  const char* argv[] = {"prog", "-size", "8", nullptr};
+  int rc = dsgl(3, (char**)argv, &cn, &new_sort, &corpus);
+  std::printf("%d %s\n", rc, dger_last_error());
+  return rc == DGER_E_VOCAB ? 0 : 1;
+}
+''')
+    libdir = os.path.dirname(G.LIB_PATH)
+    exe = tmp_path / "caller"
+    subprocess.run(["g++", "-std=c++17", str(src), "-I", os.path.join(ROOT, "include"), "-L", libdir,
+                    "-ldger_sgns", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr

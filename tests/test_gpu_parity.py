@@ -1,0 +1,147 @@
+"""GPU parity: the gfx950 trainer against the oracle, through the C ABI.
+
+Deterministic mode (one worker) must reproduce the oracle bit for bit: the
+index stream (targets, window draws, contexts, negatives, multiplicities,
+learning rate) and the final FP32 Wih / Woh.  With several workers the index
+stream of every worker is still bit-exact (it does not depend on the model);
+the model is Hogwild-updated and is checked by quality instead."""
+import numpy as np
+import pytest
+
+import dger_sgns as G
+from oracle import oracle as O
+from corpora import random_corpus
+
+pytestmark = pytest.mark.gpu
+
+TS = 10_000_000
+
+
+def gpu_opts(D, W, K, sample, iter=1, precision="fp32", alpha=0.025, ts=TS):
+    return G.default_opts(size=D, window=W, negative=K, sample=sample, alpha=alpha, iter=iter,
+                          threads=2, table_size=ts, precision=precision, debug=0)
+
+
+def orc_opts(D, W, K, sample, iter=1, workers=1, alpha=0.025, ts=TS):
+    return O.make_opts(dim=D, window=W, negative=K, sample=sample, alpha=alpha, iter=iter,
+                       threads=2, workers=workers, table_size=ts)
+
+
+def test_tables_and_init_bit_exact():
+    corpus, cn = random_corpus(1, V=3000, zipf=True)
+    with G.Trainer(gpu_opts(128, 5, 5, 1e-3, ts=100_000_000), cn, corpus) as tr:
+        rank_id, tw = tr.vocab()
+        orank, _, otw = O.build_vocab(cn)
+        assert np.array_equal(rank_id, orank) and tw == otw
+        assert np.array_equal(tr.table(), O.unigram_table(cn[orank], 100_000_000, 2))
+        wih, woh = tr.get_model()
+        assert np.array_equal(wih, O.init_net(cn.size, 128)) and not woh.any()
+
+
+CASES = [
+    # D, W, K, sample, iter, corpus kwargs
+    (128, 10, 5, 1e-3, 1, dict(seed=2, lmax=80)),
+    (64, 3, 1, 0.0, 2, dict(seed=3, lmin=1, empty_walks=5)),
+    (100, 5, 5, 1e-2, 1, dict(seed=4, zipf=True, V=1200)),
+    (256, 2, 15, 1e-3, 1, dict(seed=5, n_walks=300)),
+    (32, 1, 0, 1e-3, 1, dict(seed=6)),
+    (128, 10, 5, 1e-3, 1, dict(seed=7, n_walks=120, long_walk=2600)),
+]
+
+
+@pytest.mark.parametrize("D,W,K,sample,it,ck", CASES)
+def test_deterministic_mode_bit_exact(D, W, K, sample, it, ck):
+    corpus, cn = random_corpus(**ck)
+    print(f"case D={D} W={W} K={K} sample={sample} iter={it} tokens={corpus.size}", flush=True)
+    wih, woh, st, rec, ws = O.train(corpus, cn, orc_opts(D, W, K, sample, it), rec_cap=60000)
+    with G.Trainer(gpu_opts(D, W, K, sample, it), cn, corpus) as tr:
+        gst, grec, gws = tr.train(workers=1, rec_cap=60000)
+        gwih, gwoh = tr.get_model()
+    assert gst.steps == st.steps and gst.raw_words == st.raw_words and gst.sentences == st.sentences
+    n = int(st.steps)
+    assert n > 100
+    assert np.array_equal(grec[0, :n], rec[0, :n])
+    assert np.float32(gst.final_alpha) == np.float32(st.final_alpha)
+    assert np.array_equal(gwih, wih)
+    assert np.array_equal(gwoh, woh)
+
+
+def test_multi_worker_index_streams_bit_exact():
+    corpus, cn = random_corpus(8, n_walks=1500, zipf=True)
+    nw = 7
+    _, _, st, rec, ws = O.train(corpus, cn, orc_opts(128, 5, 5, 1e-3, workers=nw), rec_cap=20000)
+    with G.Trainer(gpu_opts(128, 5, 5, 1e-3), cn, corpus) as tr:
+        gst, grec, gws = tr.train(workers=nw, rec_cap=20000)
+    assert np.array_equal(gws, ws)
+    assert gst.steps == st.steps and gst.raw_words == st.raw_words
+    idx = np.ones(rec.shape[2], bool)
+    idx[6] = False  # the learning rate reads the shared word counter (racy across workers)
+    for w in range(nw):
+        n = int(ws[w])
+        assert np.array_equal(grec[w, :n][:, idx], rec[w, :n][:, idx]), w
+
+
+def test_bf16_form_same_index_stream_close_model():
+    corpus, cn = random_corpus(9, n_walks=800)
+    wih, woh, st, rec, ws = O.train(corpus, cn, orc_opts(128, 5, 5, 1e-3), rec_cap=30000)
+    with G.Trainer(gpu_opts(128, 5, 5, 1e-3, precision="bf16"), cn, corpus) as tr:
+        gst, grec, _ = tr.train(workers=1, rec_cap=30000)
+        gwih, gwoh = tr.get_model()
+    n = int(st.steps)
+    assert np.array_equal(grec[0, :n], rec[0, :n])
+    # BF16 rows (8 mantissa bits) with FP32 working buffers: each stored value
+    # carries at most a few bf16 roundings of the FP32 trajectory.
+    for g, r in ((gwih, wih), (gwoh, woh)):
+        err = np.abs(g - r)
+        assert np.max(err) <= 0.02 * np.max(np.abs(r))
+        assert np.linalg.norm(g - r) <= 0.01 * np.linalg.norm(r)
+
+
+def test_step_kernel_matches_oracle():
+    rng = np.random.default_rng(12)
+    t = O.exp_table()
+    for os_, nin, fin, D in [(7, 40, 21, 128), (3, 5, 5, 64), (6, 13, 0, 100), (17, 64, 30, 256)]:
+        Om = rng.normal(0, 0.4, (os_, D)).astype(np.float32)
+        Im = rng.normal(0, 0.4, (nin, D)).astype(np.float32)
+        meta = rng.integers(1, 3, os_).astype(np.int32)
+        meta[0] = meta[-1] = 1
+        corr, md, gin = G.step(Om, Im, fin, meta, 0.0125)
+        c32, md32, gin32, _ = O.step_f32(Om, Im, fin, meta, 0.0125, t)
+        assert np.array_equal(corr, c32) and np.array_equal(md, md32) and np.array_equal(gin, gin32)
+        c64, md64, gin64, s64 = O.step_f64(Om, Im, fin, meta, 0.0125, t)
+        _, _, _, s32 = O.step_f32(Om, Im, fin, meta, 0.0125, t)
+        flips = s32 != s64
+        assert flips.mean() < 0.01
+        if not flips.any():
+            # per-step update within 1e-5 relative of the FP64 oracle
+            assert np.max(np.abs(md - md64)) <= 1e-5 * np.max(np.abs(md64))
+            assert np.max(np.abs(gin - gin64)) <= 1e-5 * np.max(np.abs(gin64))
+
+
+def test_corpus_id_out_of_range():
+    cn = np.ones(20, np.int32)
+    with G.Trainer(gpu_opts(16, 2, 2, 0.0), cn) as tr:
+        with pytest.raises(G.DgerError) as e:
+            tr.set_corpus(np.array([1, 2, 20, -1], np.int32))
+        assert e.value.code == G.DGER_E_CORPUS
+        with pytest.raises(G.DgerError) as e:
+            tr.train()
+        assert e.value.code == G.DGER_E_STATE
+
+
+def test_dsgl_main_writes_reference_format(tmp_path):
+    corpus, cn = random_corpus(10, V=1100, n_walks=200)
+    out = tmp_path / "emb.txt"
+    G.dsgl(["prog", "-size", "16", "-window", "3", "-negative", "2", "-iter", "1", "-alpha", "0.025",
+            "-workers", "1", "-threads", "2", "-table-size", "1000000", "-debug", "0",
+            "-eoutput", str(out)], cn, corpus)
+    lines = out.read_text().split("\n")
+    assert lines[0] == "1100 16"
+    opts = orc_opts(16, 3, 2, 1e-3, ts=1_000_000)
+    wih, _, _, _, _ = O.train(corpus, cn, opts)
+    rank_id, _, _ = O.build_vocab(cn)
+    for r in (0, 1, 537, 1099):
+        parts = lines[1 + r].split(" ")
+        assert int(parts[0]) == rank_id[r]
+        assert parts[1:-1] == ["%f" % v for v in wih[r]]
+        assert parts[-1] == ""
