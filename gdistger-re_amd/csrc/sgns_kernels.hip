@@ -85,153 +85,227 @@ struct Row<uint16_t> {
     *p = f32_to_bf16(v.x) | (f32_to_bf16(v.y) << 16);
   }
   __device__ static void atomic_add(void* base, size_t row, int D, int pr, float2 v) {
-    unsigned* p = reinterpret_cast<unsigned*>(static_cast<uint16_t*>(base) + row * (size_t)D) + pr;
-    unsigned old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (;;) {
-      float x = bf16_to_f32(old & 0xFFFFu) + v.x;
-      float y = bf16_to_f32(old >> 16) + v.y;
-      unsigned nv = f32_to_bf16(x) | (f32_to_bf16(y) << 16);
-      unsigned seen = old;
-      if (__hip_atomic_compare_exchange_strong(p, &seen, nv, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT))
-        break;
-      old = seen;
-    }
+    typedef short __attribute__((ext_vector_type(2))) short2v;
+    short2v* p = reinterpret_cast<short2v*>(static_cast<uint16_t*>(base) + row * (size_t)D) + pr;
+    short2v x;
+    x.x = (short)f32_to_bf16(v.x);
+    x.y = (short)f32_to_bf16(v.y);
+    (void)__builtin_amdgcn_flat_atomic_fadd_v2bf16(p, x);
   }
 };
 
-// Table-quantised sigmoid gradient, dsgl.hpp:1249-1270.
+// Table-quantised sigmoid gradient, dsgl.hpp:1249-1270.  The index is
+// clamped to the table only so that an undefined input can never read out of
+// bounds; for every finite f in [-6, 6] it is (int)((f + 6) * 83) <= 996.
 __device__ __forceinline__ float sig_grad(float f, int label, float alpha, const float* exp_table) {
   if (f > 6.0f) return (float)(label - 1) * alpha;
   if (f < -6.0f) return (float)label * alpha;
   int idx = (int)((f + 6.0f) * 83.0f);
+  idx = min(max(idx, 0), kExpTable - 1);
   return ((float)label - exp_table[idx]) * alpha;
 }
 
+constexpr int kOC = 8;  // output rows per register chunk
+
 // LDS carve-up of one worker (all offsets 16-byte aligned).
 struct Lds {
-  int* tok[2];    // Lmax ranks per sentence
+  int* tok[2];    // Lmax ranks per sentence (tok[1] == tok[0] + Lmax)
   int* slot[2];   // Lmax: first position holding the same rank
   int* negw;      // K*Lmax negative draws
   int* negs;      // K*Lmax: first draw holding the same rank
   int* cpos;      // 4W context positions
   int* csent;     // 4W context sentence (0/1)
   int* crep;      // 4W first context of the step with the same buffer row
-  int* oidx;      // K+2 output ranks
-  int* ometa;     // K+2 multiplicities
-  int* oslot;     // K+2 negative-buffer rows
+  int* ckey;      // 4W buffer row of the context: sentence*Lmax + slot
+  int* oidx;      // KO8 output ranks
+  int* ometa;     // KO8 multiplicities
+  int* oslot;     // KO8 negative-buffer rows
   int* misc;      // 8 scalars
-  float* O;       // (K+2) x OST staged output rows
+  float* O;       // KO8 x OST staged output rows
   float* I;       // 4W x IST staged context rows
-  float* C;       // (K+2) x 4W gradient coefficients
+  float* Ct;      // 4W x CST gradient coefficients, transposed: Ct[j][i]
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+__host__ __device__ inline int ko8(int K) { return ((K + 2) + kOC - 1) & ~(kOC - 1); }
 __host__ __device__ inline int o_stride(int D) { return (D + 3) & ~3; }
-__host__ __device__ inline int i_stride(int D) { return D + 1; }
+__host__ __device__ inline int i_stride(int D) { return ((D + 3) & ~3) + 4; }
+__host__ __device__ inline int c_stride(int K) { return ko8(K); }
 
 __host__ __device__ inline size_t lds_layout(int Lmax, int K, int W, int D, size_t* off) {
-  const int KO = K + 2, MC = 4 * W;
+  const int KO = ko8(K), MC = 4 * W;
   size_t o = 0;
   off[0] = o; o = align16(o + 4 * (size_t)Lmax * 4);      // tok0, tok1, slot0, slot1
   off[1] = o; o = align16(o + 2 * (size_t)K * Lmax * 4);  // negw, negs
-  off[2] = o; o = align16(o + 3 * (size_t)MC * 4);        // cpos, csent, crep
+  off[2] = o; o = align16(o + 4 * (size_t)MC * 4);        // cpos, csent, crep, ckey
   off[3] = o; o = align16(o + 3 * (size_t)KO * 4 + 8 * 4);// oidx, ometa, oslot, misc
   off[4] = o; o = align16(o + (size_t)KO * o_stride(D) * 4);
   off[5] = o; o = align16(o + (size_t)MC * i_stride(D) * 4);
-  off[6] = o; o = align16(o + (size_t)KO * MC * 4);
+  off[6] = o; o = align16(o + (size_t)MC * c_stride(K) * 4);
   return o;
 }
 
 __device__ inline Lds carve(unsigned char* smem, int Lmax, int K, int W, int D) {
   size_t off[7];
   lds_layout(Lmax, K, W, D, off);
-  const int KO = K + 2, MC = 4 * W;
+  const int KO = ko8(K), MC = 4 * W;
   Lds s;
   int* b0 = reinterpret_cast<int*>(smem + off[0]);
   s.tok[0] = b0; s.tok[1] = b0 + Lmax; s.slot[0] = b0 + 2 * Lmax; s.slot[1] = b0 + 3 * Lmax;
   int* b1 = reinterpret_cast<int*>(smem + off[1]);
   s.negw = b1; s.negs = b1 + K * Lmax;
   int* b2 = reinterpret_cast<int*>(smem + off[2]);
-  s.cpos = b2; s.csent = b2 + MC; s.crep = b2 + 2 * MC;
+  s.cpos = b2; s.csent = b2 + MC; s.crep = b2 + 2 * MC; s.ckey = b2 + 3 * MC;
   int* b3 = reinterpret_cast<int*>(smem + off[3]);
   s.oidx = b3; s.ometa = b3 + KO; s.oslot = b3 + 2 * KO; s.misc = b3 + 3 * KO;
   s.O = reinterpret_cast<float*>(smem + off[4]);
   s.I = reinterpret_cast<float*>(smem + off[5]);
-  s.C = reinterpret_cast<float*>(smem + off[6]);
+  s.Ct = reinterpret_cast<float*>(smem + off[6]);
   return s;
 }
 
 // first[p] = smallest q <= p with v[q] == v[p], for p < n (values in LDS).
+// The inner loop has a wave-uniform trip count and broadcast reads.
 __device__ inline void first_occurrence(const int* v, int* first, int n, int lane) {
   for (int c0 = 0; c0 < n; c0 += 64) {
-    int p = c0 + lane;
-    if (p < n) {
-      int x = v[p];
-      int f = p;
-      for (int q = 0; q < p; q++) {
-        if (v[q] == x) { f = q; break; }
-      }
-      first[p] = f;
+    const int p = c0 + lane;
+    const int x = (p < n) ? v[p] : 0;
+    const int lim = min(n, c0 + 64);
+    int f = p;
+#pragma unroll 8
+    for (int q = 0; q < lim; q++) {
+      const int y = v[q];
+      f = (q < f && y == x) ? q : f;
     }
+    if (p < n) first[p] = f;
   }
 }
 
 // GEMM1 + sigmoid (dsgl.hpp:1241-1272): lanes over contexts j; for each
-// output row i, f = sum_k O[i][k]*I[j][k] in sequential k.  C[i][j] = g * meta.
-__device__ inline void step_gemm1(const Lds& s, int D, int MC, int os, int nin, int fin, float alpha,
+// chunk of 8 output rows, f[i] = sum_k O[i][k]*I[j][k] in sequential k with
+// one FMA per term.  Ct[j][i] = g * meta[i] for i < os, 0 for padding rows.
+__device__ inline void step_gemm1(const Lds& s, int D, int K, int os, int nin, int fin, float alpha,
                                   const float* exp_table, int lane) {
-  const int OST = o_stride(D), IST = i_stride(D);
+  const int OST = o_stride(D), IST = i_stride(D), CST = c_stride(K);
+  const int D4 = D >> 2;
   for (int j = lane; j < nin; j += 64) {
     const float* x = s.I + (size_t)j * IST;
-    for (int i0 = 0; i0 < os; i0 += 8) {
-      const int ni = min(8, os - i0);
-      float acc[8];
+    for (int i0 = 0; i0 < os; i0 += kOC) {
+      float acc[kOC];
 #pragma unroll
-      for (int ii = 0; ii < 8; ii++) acc[ii] = 0.f;
-      for (int k = 0; k < D; k++) {
+      for (int ii = 0; ii < kOC; ii++) acc[ii] = 0.f;
+      for (int k4 = 0; k4 < D4; k4++) {
+        const float4 xv = *reinterpret_cast<const float4*>(x + 4 * k4);
+#pragma unroll
+        for (int ii = 0; ii < kOC; ii++) {
+          const float4 ov = *reinterpret_cast<const float4*>(s.O + (i0 + ii) * OST + 4 * k4);
+          acc[ii] = __builtin_fmaf(ov.x, xv.x, acc[ii]);
+          acc[ii] = __builtin_fmaf(ov.y, xv.y, acc[ii]);
+          acc[ii] = __builtin_fmaf(ov.z, xv.z, acc[ii]);
+          acc[ii] = __builtin_fmaf(ov.w, xv.w, acc[ii]);
+        }
+      }
+      for (int k = 4 * D4; k < D; k++) {
         const float xv = x[k];
 #pragma unroll
-        for (int ii = 0; ii < 8; ii++)
-          if (ii < ni) acc[ii] = __builtin_fmaf(s.O[(i0 + ii) * OST + k], xv, acc[ii]);
+        for (int ii = 0; ii < kOC; ii++) acc[ii] = __builtin_fmaf(s.O[(i0 + ii) * OST + k], xv, acc[ii]);
       }
 #pragma unroll
-      for (int ii = 0; ii < 8; ii++) {
-        if (ii < ni) {
-          const int i = i0 + ii;
-          const int label = (j < fin) ? (i == 0 ? 1 : 0) : (i == os - 1 ? 1 : 0);
-          const float g = sig_grad(acc[ii], label, alpha, exp_table);
-          s.C[i * MC + j] = g * (float)s.ometa[i];
+      for (int ii = 0; ii < kOC; ii++) {
+        const int i = i0 + ii;
+        const int label = (j < fin) ? (i == 0 ? 1 : 0) : (i == os - 1 ? 1 : 0);
+        const float g = sig_grad(acc[ii], label, alpha, exp_table) * (float)s.ometa[i];
+        s.Ct[(size_t)j * CST + i] = (i < os) ? g : 0.f;
+      }
+    }
+  }
+}
+
+// GEMM2 (dsgl.hpp:1290): md[i] = sum_j C[i][j] * I[j][:] in sequential j;
+// lanes over element pairs, 8 output rows per chunk; sink(i, pr, md) for i < os.
+template <int NP, typename Sink>
+__device__ inline void step_gemm2(const Lds& s, int D, int K, int os, int nin, int lane, Sink&& sink) {
+  const int IST = i_stride(D), CST = c_stride(K), DP = D >> 1;
+  for (int i0 = 0; i0 < os; i0 += kOC) {
+    float2 acc[kOC][NP];
+#pragma unroll
+    for (int ii = 0; ii < kOC; ii++)
+#pragma unroll
+      for (int u = 0; u < NP; u++) acc[ii][u] = make_float2(0.f, 0.f);
+    for (int j = 0; j < nin; j++) {
+      const float4 c0 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0);
+      const float4 c1 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0 + 4);
+      const float c[kOC] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+      for (int u = 0; u < NP; u++) {
+        const int pr = lane + 64 * u;
+        const float2 x = (pr < DP) ? *reinterpret_cast<const float2*>(s.I + (size_t)j * IST + 2 * pr)
+                                   : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int ii = 0; ii < kOC; ii++) {
+          acc[ii][u].x = __builtin_fmaf(c[ii], x.x, acc[ii][u].x);
+          acc[ii][u].y = __builtin_fmaf(c[ii], x.y, acc[ii][u].y);
+        }
+      }
+    }
+#pragma unroll
+    for (int ii = 0; ii < kOC; ii++) {
+      const int i = i0 + ii;
+      if (i < os) {
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          if (pr < DP) sink(i, pr, acc[ii][u]);
         }
       }
     }
   }
 }
 
-// GEMM2 for output row i (dsgl.hpp:1290): md = sum_j C[i][j] * I[j][:] in
-// sequential j, for the element pair pr of this lane.
-__device__ inline float2 step_gemm2_row(const Lds& s, int D, int MC, int i, int nin, int pr) {
-  const int IST = i_stride(D);
-  float ax = 0.f, ay = 0.f;
+// GEMM3 (dsgl.hpp:1309): gin[j] = sum_i C[i][j] * O[i][:] in sequential i;
+// lanes over element pairs; sink(j, pr, gin) for every context j in order.
+template <int NP, typename Sink>
+__device__ inline void step_gemm3(const Lds& s, int D, int K, int os, int nin, int lane, Sink&& sink) {
+  const int OST = o_stride(D), CST = c_stride(K), DP = D >> 1;
+  float2 o0[kOC][NP];  // output rows 0..7 stay in registers
+#pragma unroll
+  for (int ii = 0; ii < kOC; ii++)
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      const int pr = lane + 64 * u;
+      o0[ii][u] = (pr < DP) ? *reinterpret_cast<const float2*>(s.O + ii * OST + 2 * pr) : make_float2(0.f, 0.f);
+    }
   for (int j = 0; j < nin; j++) {
-    const float c = s.C[i * MC + j];
-    ax = __builtin_fmaf(c, s.I[(size_t)j * IST + 2 * pr], ax);
-    ay = __builtin_fmaf(c, s.I[(size_t)j * IST + 2 * pr + 1], ay);
+    float2 g[NP];
+#pragma unroll
+    for (int u = 0; u < NP; u++) g[u] = make_float2(0.f, 0.f);
+    for (int i0 = 0; i0 < os; i0 += kOC) {
+      const float4 c0 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0);
+      const float4 c1 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0 + 4);
+      const float c[kOC] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+      for (int ii = 0; ii < kOC; ii++) {
+        const bool on = i0 + ii < os;
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          float2 o = o0[ii][u];
+          if (i0 > 0)
+            o = (pr < DP) ? *reinterpret_cast<const float2*>(s.O + (i0 + ii) * OST + 2 * pr)
+                          : make_float2(0.f, 0.f);
+          const float gx = __builtin_fmaf(c[ii], o.x, g[u].x);
+          const float gy = __builtin_fmaf(c[ii], o.y, g[u].y);
+          g[u].x = on ? gx : g[u].x;
+          g[u].y = on ? gy : g[u].y;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      const int pr = lane + 64 * u;
+      if (pr < DP) sink(j, pr, g[u]);
+    }
   }
-  return make_float2(ax, ay);
-}
-
-// GEMM3 for context j (dsgl.hpp:1309): gin = sum_i C[i][j] * O[i][:] in
-// sequential i, for the element pair pr of this lane.
-__device__ inline float2 step_gemm3_col(const Lds& s, int D, int MC, int j, int os, int pr) {
-  const int OST = o_stride(D);
-  float ax = 0.f, ay = 0.f;
-  for (int i = 0; i < os; i++) {
-    const float c = s.C[i * MC + j];
-    ax = __builtin_fmaf(c, s.O[i * OST + 2 * pr], ax);
-    ay = __builtin_fmaf(c, s.O[i * OST + 2 * pr + 1], ay);
-  }
-  return make_float2(ax, ay);
 }
 
 __device__ __forceinline__ float2 ld2(const float* p) { return *reinterpret_cast<const float2*>(p); }
@@ -246,6 +320,13 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
   const int DP = D >> 1;
   const int MC = 4 * W, IST = i_stride(D), OST = o_stride(D);
   const Lds s = carve(smem, Lmax, K, W, D);
+  int* const tok = s.tok[0];    // sentence q: tok + q*Lmax
+  int* const slot = s.slot[0];  // sentence q: slot + q*Lmax
+  constexpr int B = (16 / NP) > 2 ? (16 / NP) : 2;  // rows with loads in flight per batch
+  // Output rows beyond the current step's count are read (and their results
+  // discarded) by the 8-row GEMM chunks: start them finite.
+  for (int x = lane; x < ko8(K) * OST; x += 64) s.O[x] = 0.f;
+  __syncthreads();
 
   // Corpus slice and seed of compute thread wk+1 (dsgl.hpp:791, :811-821).
   const u64 N = a.n_tokens;
@@ -261,19 +342,78 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
   u64 jA = 1, jC = 0;
   for (int i = 0; i <= lane; i++) { jA = jA * kLcgMul; jC = jC * kLcgMul + kLcgAdd; }
 
-  float* const buf[2] = {a.fscratch + (size_t)wk * worker_scratch_floats(Lmax, K, D),
-                         a.fscratch + (size_t)wk * worker_scratch_floats(Lmax, K, D) + (size_t)Lmax * D};
-  float* const nbuf = buf[1] + (size_t)Lmax * D;
+  // Working buffers (dsgl.hpp:844-857): sentence q rows at buf + q*Lmax*D,
+  // indexed by the first position of each rank; negative rows at nbuf,
+  // indexed by the first draw of each rank.
+  float* const buf = a.fscratch + (size_t)wk * worker_scratch_floats(Lmax, K, D);
+  float* const nbuf = buf + 2 * (size_t)Lmax * D;
 
   const float a0 = a.alpha0;
   float alpha = a0;
   int iters_left = a.iter;
   u64 wc = 0, last_wc = 0, cur = start;
-  int len[2] = {0, 0}, pos[2] = {0, 0};
+  int len0 = 0, len1 = 0, pos0 = 0, pos1 = 0;
   int done_sent = 0, refresh = 1, neg_count = 0, nbn = 1;
   u64 steps = 0, raw = 0, sentences = 0;
   const int stride = record_stride(W, K);
   int32_t* const rec = a.rec ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
+
+  // Copies model rows src_rows[ids[k]] to dst + k*D for the k < n with
+  // first[k] == k (the first occurrence of each rank), B rows per batch.
+  auto gather = [&](const void* src_rows, const int* ids, const int* first, int n, float* dst) {
+    for (int k0 = 0; k0 < n; k0 += B) {
+      float2 v[B][NP];
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        const int k = k0 + b;
+        const bool on = k < n && first[k] == k;
+        const size_t row = on ? (size_t)ids[k] : 0;
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          if (on && pr < DP) v[b][u] = Row<R>::load(src_rows, row, D, pr);
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        const int k = k0 + b;
+        const bool on = k < n && first[k] == k;
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          if (on && pr < DP) st2(dst + (size_t)k * D + 2 * pr, v[b][u]);
+        }
+      }
+    }
+  };
+  // The reverse copy: working rows back over the model rows (the memcpy
+  // write-backs of dsgl.hpp:1377-1407).
+  auto scatter = [&](void* dst_rows, const int* ids, const int* first, int n, const float* src) {
+    for (int k0 = 0; k0 < n; k0 += B) {
+      float2 v[B][NP];
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        const int k = k0 + b;
+        const bool on = k < n && first[k] == k;
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          if (on && pr < DP) v[b][u] = ld2(src + (size_t)k * D + 2 * pr);
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        const int k = k0 + b;
+        const bool on = k < n && first[k] == k;
+        const size_t row = on ? (size_t)ids[k] : 0;
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          if (on && pr < DP) Row<R>::store(dst_rows, row, D, pr, v[b][u]);
+        }
+      }
+    }
+  };
 
   for (;;) {
     // Learning-rate decay in >10000-token chunks (dsgl.hpp:886-897).
@@ -289,7 +429,10 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
 
     // Sentence loads (dsgl.hpp:902-965, :968-1030).
     for (int q = 0; q < 2; q++) {
-      while (len[q] == 0) {
+      int* const tq = tok + q * Lmax;
+      int* const sq = slot + q * Lmax;
+      for (;;) {
+        if ((q ? len1 : len0) != 0) break;
         if (cur >= end) break;
         int L = 0;
         for (;;) {
@@ -308,7 +451,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
               keep = !(a.keep_thr[r] < (float)(unsigned)(sr & 0xFFFFu) / 65536.f);
             }
           }
-          u64 km = __ballot(keep);
+          const u64 km = __ballot(keep);
           int nk = __popcll(km);
           const int room = kMaxSentence - L;
           int consumed, draws;
@@ -323,29 +466,19 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
           } else {
             consumed = 64; draws = 64; stop = false;
           }
-          if (keep) s.tok[q][L + __popcll(km & lanemask_lt(lane))] = r;
+          if (keep) tq[L + __popcll(km & lanemask_lt(lane))] = r;
           L += nk;
           if (a.sample_on && draws > 0) rng = readlane64(jA, draws - 1) * rng + readlane64(jC, draws - 1);
           cur += (u64)consumed;
           wc += (u64)consumed;
           if (stop) break;
         }
-        pos[q] = 0;
-        if (L <= 1) { len[q] = 0; continue; }
-        len[q] = L;
+        if (q) { pos1 = 0; len1 = L > 1 ? L : 0; } else { pos0 = 0; len0 = L > 1 ? L : 0; }
+        if (L <= 1) continue;
         __syncthreads();
-        first_occurrence(s.tok[q], s.slot[q], L, lane);
+        first_occurrence(tq, sq, L, lane);
         __syncthreads();
-        // Working copy of the sentence's Wih rows, one per distinct rank.
-        for (int p = 0; p < L; p++) {
-          if (s.slot[q][p] != p) continue;
-          const size_t row = (size_t)s.tok[q][p];
-#pragma unroll
-          for (int v = 0; v < NP; v++) {
-            const int pr = lane + 64 * v;
-            if (pr < DP) st2(buf[q] + (size_t)p * D + 2 * pr, Row<R>::load(a.wih, row, D, pr));
-          }
-        }
+        gather(a.wih, tq, sq, L, buf + (size_t)q * Lmax * D);
       }
     }
 
@@ -355,13 +488,13 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       raw += wc;
       if (--iters_left == 0) break;
       wc = 0; cur = start; last_wc = 0;
-      len[0] = 0; len[1] = 0;
+      len0 = 0; len1 = 0;
       continue;
     }
 
     // Shared negative batch (dsgl.hpp:1054-1088).
     if (refresh) {
-      nbn = max(len[0], len[1]);
+      nbn = max(len0, len1);
       const int NN = nbn * K;
       u64 s0 = rng;
       for (int base = 0; base < NN; base += 64) {
@@ -378,34 +511,27 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       __syncthreads();
       first_occurrence(s.negw, s.negs, NN, lane);
       __syncthreads();
-      for (int n = 0; n < NN; n++) {
-        if (s.negs[n] != n) continue;
-        const size_t row = (size_t)s.negw[n];
-#pragma unroll
-        for (int v = 0; v < NP; v++) {
-          const int pr = lane + 64 * v;
-          if (pr < DP) st2(nbuf + (size_t)n * D + 2 * pr, Row<R>::load(a.woh, row, D, pr));
-        }
-      }
+      gather(a.woh, s.negw, s.negs, NN, nbuf);
       refresh = 0;
     }
 
     // ---- one step (dsgl.hpp:1089-1368) ----
-    const int t1 = s.tok[0][pos[0]];
-    const int t2 = s.tok[1][pos[1]];
+    const int t1 = tok[pos0];
+    const int t2 = tok[Lmax + pos1];
     rng = rng * kLcgMul + kLcgAdd;
     const int b = (int)(rng % (u64)W);
     const int hw = W - b;
-    const int lo0 = max(0, pos[0] - hw), hi0 = min(len[0] - 1, pos[0] + hw);
-    const int lo1 = max(0, pos[1] - hw), hi1 = min(len[1] - 1, pos[1] + hw);
-    const int n0 = (pos[0] - lo0) + (hi0 - pos[0]);
-    const int n1 = (pos[1] - lo1) + (hi1 - pos[1]);
+    const int lo0 = max(0, pos0 - hw), hi0 = min(len0 - 1, pos0 + hw);
+    const int lo1 = max(0, pos1 - hw), hi1 = min(len1 - 1, pos1 + hw);
+    const int n0 = (pos0 - lo0) + (hi0 - pos0);
+    const int n1 = (pos1 - lo1) + (hi1 - pos1);
     const int nin = n0 + n1, fin = n0;
     for (int j = lane; j < nin; j += 64) {
       const int q = (j < n0) ? 0 : 1;
       const int jj = q ? j - n0 : j;
-      const int lo = q ? lo1 : lo0, pp = pos[q];
-      s.cpos[j] = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
+      const int lo = q ? lo1 : lo0, pp = q ? pos1 : pos0;
+      const int c = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
+      s.cpos[j] = c;
       s.csent[j] = q;
     }
     neg_count++;
@@ -433,41 +559,60 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
     }
     __syncthreads();
     const int os = s.misc[0];
-    // Repeat contexts of a step share one buffer row: crep[j] = first such j.
+    // Buffer row of each context, and the first context of the step that
+    // shares it (repeat contexts reduce into that row, in context order).
     for (int j = lane; j < nin; j += 64) {
       const int q = s.csent[j];
-      const int sl = s.slot[q][s.cpos[j]];
-      int r = j;
-      for (int j2 = (q ? n0 : 0); j2 < j; j2++)
-        if (s.slot[q][s.cpos[j2]] == sl) { r = j2; break; }
-      s.crep[j] = r;
-    }
-    // Stage output rows (targets from Woh, negatives from the negative buffer)
-    // and context rows (from the sentence buffers).
-    for (int i = 0; i < os; i++) {
-#pragma unroll
-      for (int v = 0; v < NP; v++) {
-        const int pr = lane + 64 * v;
-        if (pr < DP) {
-          float2 x;
-          if (i == 0) x = Row<R>::load(a.woh, (size_t)t1, D, pr);
-          else if (i == os - 1) x = Row<R>::load(a.woh, (size_t)t2, D, pr);
-          else x = ld2(nbuf + (size_t)s.oslot[i] * D + 2 * pr);
-          st2(s.O + i * OST + 2 * pr, x);
-        }
-      }
+      s.ckey[j] = q * Lmax + slot[q * Lmax + s.cpos[j]];
     }
     __syncthreads();
-    for (int j = 0; j < nin; j++) {
-      const int q = s.csent[j];
-      const float* src = buf[q] + (size_t)s.slot[q][s.cpos[j]] * D;
+    for (int j0 = 0; j0 < nin; j0 += 64) {
+      const int j = j0 + lane;
+      const int key = (j < nin) ? s.ckey[j] : -1;
+      const int lim = min(nin, j0 + 64);
+      int rep = j;
+#pragma unroll 8
+      for (int j2 = 0; j2 < lim; j2++) {
+        const int k2 = s.ckey[j2];
+        rep = (j2 < rep && k2 == key) ? j2 : rep;
+      }
+      if (j < nin) s.crep[j] = rep;
+    }
+    // Stage output rows (targets from Woh, negatives from the negative
+    // buffer) and context rows (from the sentence buffers), B rows in flight.
+    const int nrows = os + nin;
+    for (int r0 = 0; r0 < nrows; r0 += B) {
+      float2 v[B][NP];
 #pragma unroll
-      for (int v = 0; v < NP; v++) {
-        const int pr = lane + 64 * v;
-        if (pr < DP) {
-          const float2 x = ld2(src + 2 * pr);
-          s.I[(size_t)j * IST + 2 * pr] = x.x;
-          s.I[(size_t)j * IST + 2 * pr + 1] = x.y;
+      for (int bb = 0; bb < B; bb++) {
+        const int r = r0 + bb;
+        if (r < nrows) {
+          const float* src = nullptr;
+          size_t mrow = 0;
+          if (r < os) {
+            if (r == 0) mrow = (size_t)t1;
+            else if (r == os - 1) mrow = (size_t)t2;
+            else src = nbuf + (size_t)s.oslot[r] * D;
+          } else {
+            src = buf + (size_t)s.ckey[r - os] * D;
+          }
+#pragma unroll
+          for (int u = 0; u < NP; u++) {
+            const int pr = lane + 64 * u;
+            if (pr < DP) v[bb][u] = src ? ld2(src + 2 * pr) : Row<R>::load(a.woh, mrow, D, pr);
+          }
+        }
+      }
+#pragma unroll
+      for (int bb = 0; bb < B; bb++) {
+        const int r = r0 + bb;
+        if (r < nrows) {
+          float* dst = (r < os) ? s.O + r * OST : s.I + (size_t)(r - os) * IST;
+#pragma unroll
+          for (int u = 0; u < NP; u++) {
+            const int pr = lane + 64 * u;
+            if (pr < DP) st2(dst + 2 * pr, v[bb][u]);
+          }
         }
       }
     }
@@ -479,90 +624,61 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       r[0] = t1; r[1] = t2; r[2] = b; r[3] = fin; r[4] = nin; r[5] = os;
       r[6] = __float_as_int(alpha);
       r[7] = grp;
-      for (int j = 0; j < nin; j++) r[8 + j] = s.tok[s.csent[j]][s.cpos[j]];
+      for (int j = 0; j < nin; j++) r[8 + j] = tok[s.csent[j] * Lmax + s.cpos[j]];
       for (int i = 0; i < os; i++) { r[8 + MC + i] = s.oidx[i]; r[8 + MC + K + 2 + i] = s.ometa[i]; }
     }
 
-    step_gemm1(s, D, MC, os, nin, fin, alpha, a.exp_table, lane);
+    step_gemm1(s, D, K, os, nin, fin, alpha, a.exp_table, lane);
     __syncthreads();
 
     // GEMM2 and the output-row scatter: target rows straight into Woh
     // (row 0 first, then row os-1, as dsgl.hpp:1352-1356), negatives into the
-    // negative buffer (dsgl.hpp:1358-1366).
-    for (int i = 0; i < os; i++) {
-#pragma unroll
-      for (int v = 0; v < NP; v++) {
-        const int pr = lane + 64 * v;
-        if (pr < DP) {
-          const float2 md = step_gemm2_row(s, D, MC, i, nin, pr);
-          if (i == 0) Row<R>::atomic_add(a.woh, (size_t)t1, D, pr, md);
-          else if (i == os - 1) Row<R>::atomic_add(a.woh, (size_t)t2, D, pr, md);
-          else {
-            float* dst = nbuf + (size_t)s.oslot[i] * D + 2 * pr;
-            float2 x = ld2(dst);
-            x.x += md.x; x.y += md.y;
-            st2(dst, x);
-          }
-        }
+    // negative buffer (dsgl.hpp:1358-1366) from their staged values.
+    step_gemm2<NP>(s, D, K, os, nin, lane, [&](int i, int pr, float2 md) {
+      if (i == 0) Row<R>::atomic_add(a.woh, (size_t)t1, D, pr, md);
+      else if (i == os - 1) Row<R>::atomic_add(a.woh, (size_t)t2, D, pr, md);
+      else {
+        const float2 o = ld2(s.O + i * OST + 2 * pr);
+        st2(nbuf + (size_t)s.oslot[i] * D + 2 * pr, make_float2(o.x + md.x, o.y + md.y));
       }
-    }
+    });
     // GEMM3 and the context scatter: updates reduced into the first context
-    // row of each buffer row in context order (dsgl.hpp:1314-1333).
-    for (int j = 0; j < nin; j++) {
-      const int r = s.crep[j];
-#pragma unroll
-      for (int v = 0; v < NP; v++) {
-        const int pr = lane + 64 * v;
-        if (pr < DP) {
-          const float2 g = step_gemm3_col(s, D, MC, j, os, pr);
-          s.I[(size_t)r * IST + 2 * pr] += g.x;
-          s.I[(size_t)r * IST + 2 * pr + 1] += g.y;
-        }
-      }
-    }
+    // row of each buffer row in context order (dsgl.hpp:1314-1333), then one
+    // store per distinct buffer row.  Each lane owns its element pairs, so
+    // the reduction needs no barrier.
+    step_gemm3<NP>(s, D, K, os, nin, lane, [&](int j, int pr, float2 g) {
+      float* x = s.I + (size_t)s.crep[j] * IST + 2 * pr;
+      const float2 v = ld2(x);
+      st2(x, make_float2(v.x + g.x, v.y + g.y));
+    });
     for (int j = 0; j < nin; j++) {
       if (s.crep[j] != j) continue;
-      const int q = s.csent[j];
-      float* dst = buf[q] + (size_t)s.slot[q][s.cpos[j]] * D;
+      float* dst = buf + (size_t)s.ckey[j] * D;
 #pragma unroll
-      for (int v = 0; v < NP; v++) {
-        const int pr = lane + 64 * v;
-        if (pr < DP)
-          st2(dst + 2 * pr, make_float2(s.I[(size_t)j * IST + 2 * pr], s.I[(size_t)j * IST + 2 * pr + 1]));
+      for (int u = 0; u < NP; u++) {
+        const int pr = lane + 64 * u;
+        if (pr < DP) st2(dst + 2 * pr, ld2(s.I + (size_t)j * IST + 2 * pr));
       }
     }
     steps++;
 
     // Write-backs (dsgl.hpp:1377-1407).
-    pos[0]++;
-    pos[1]++;
-    for (int q = 0; q < 2; q++) {
-      if (pos[q] >= len[q]) {
-        for (int p = 0; p < len[q]; p++) {
-          if (s.slot[q][p] != p) continue;
-          const size_t row = (size_t)s.tok[q][p];
-#pragma unroll
-          for (int v = 0; v < NP; v++) {
-            const int pr = lane + 64 * v;
-            if (pr < DP) Row<R>::store(a.wih, row, D, pr, ld2(buf[q] + (size_t)p * D + 2 * pr));
-          }
-        }
-        len[q] = 0;
-        done_sent++;
-        sentences++;
-      }
+    pos0++;
+    pos1++;
+    if (pos0 >= len0) {
+      scatter(a.wih, tok, slot, len0, buf);
+      len0 = 0;
+      done_sent++;
+      sentences++;
+    }
+    if (pos1 >= len1) {
+      scatter(a.wih, tok + Lmax, slot + Lmax, len1, buf + (size_t)Lmax * D);
+      len1 = 0;
+      done_sent++;
+      sentences++;
     }
     if (done_sent >= 2) {
-      const int NN = nbn * K;
-      for (int n = 0; n < NN; n++) {
-        if (s.negs[n] != n) continue;
-        const size_t row = (size_t)s.negw[n];
-#pragma unroll
-        for (int v = 0; v < NP; v++) {
-          const int pr = lane + 64 * v;
-          if (pr < DP) Row<R>::store(a.woh, row, D, pr, ld2(nbuf + (size_t)n * D + 2 * pr));
-        }
-      }
+      scatter(a.woh, s.negw, s.negs, nbn * K, nbuf);
       refresh = 1;
       done_sent = 0;
     }
@@ -642,32 +758,22 @@ __global__ __launch_bounds__(64) void k_step(int D, int os, int nin, int fin, co
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x;
   const int K = os - 2, W = (nin + 3) / 4 > 0 ? (nin + 3) / 4 : 1;
-  const int MC = 4 * W, DP = D >> 1, OST = o_stride(D), IST = i_stride(D);
+  const int OST = o_stride(D), IST = i_stride(D), CST = c_stride(K);
   const Lds s = carve(smem, 1, K, W, D);
-  for (int i = lane; i < os; i += 64) s.ometa[i] = meta[i];
-  for (int i = 0; i < os; i++)
-    for (int d = lane; d < D; d += 64) s.O[i * OST + d] = O[(size_t)i * D + d];
+  for (int i = lane; i < ko8(K); i += 64) s.ometa[i] = i < os ? meta[i] : 0;
+  for (int i = 0; i < ko8(K); i++)
+    for (int d = lane; d < OST; d += 64) s.O[i * OST + d] = (i < os && d < D) ? O[(size_t)i * D + d] : 0.f;
   for (int j = 0; j < nin; j++)
     for (int d = lane; d < D; d += 64) s.I[(size_t)j * IST + d] = I[(size_t)j * D + d];
   __syncthreads();
-  step_gemm1(s, D, MC, os, nin, fin, alpha, exp_table, lane);
+  step_gemm1(s, D, K, os, nin, fin, alpha, exp_table, lane);
   __syncthreads();
   for (int i = 0; i < os; i++)
-    for (int j = lane; j < nin; j += 64) corr[(size_t)i * nin + j] = s.C[i * MC + j];
-  for (int i = 0; i < os; i++) {
-#pragma unroll
-    for (int v = 0; v < NP; v++) {
-      const int pr = lane + 64 * v;
-      if (pr < DP) st2(md + (size_t)i * D + 2 * pr, step_gemm2_row(s, D, MC, i, nin, pr));
-    }
-  }
-  for (int j = 0; j < nin; j++) {
-#pragma unroll
-    for (int v = 0; v < NP; v++) {
-      const int pr = lane + 64 * v;
-      if (pr < DP) st2(gin + (size_t)j * D + 2 * pr, step_gemm3_col(s, D, MC, j, os, pr));
-    }
-  }
+    for (int j = lane; j < nin; j += 64) corr[(size_t)i * nin + j] = s.Ct[(size_t)j * CST + i];
+  step_gemm2<NP>(s, D, K, os, nin, lane,
+                 [&](int i, int pr, float2 v) { st2(md + (size_t)i * D + 2 * pr, v); });
+  step_gemm3<NP>(s, D, K, os, nin, lane,
+                 [&](int j, int pr, float2 v) { st2(gin + (size_t)j * D + 2 * pr, v); });
 }
 
 // ---- SGNS logistic loss and adjoint (exact sigmoid) -------------------------
