@@ -194,6 +194,7 @@ __device__ inline void step_gemm1(const Lds& s, int D, int K, int os, int nin, i
       float acc[kOC];
 #pragma unroll
       for (int ii = 0; ii < kOC; ii++) acc[ii] = 0.f;
+#pragma unroll 2
       for (int k4 = 0; k4 < D4; k4++) {
         const float4 xv = *reinterpret_cast<const float4*>(x + 4 * k4);
 #pragma unroll
@@ -232,6 +233,7 @@ __device__ inline void step_gemm2(const Lds& s, int D, int K, int os, int nin, i
     for (int ii = 0; ii < kOC; ii++)
 #pragma unroll
       for (int u = 0; u < NP; u++) acc[ii][u] = make_float2(0.f, 0.f);
+#pragma unroll 2
     for (int j = 0; j < nin; j++) {
       const float4 c0 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0);
       const float4 c1 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0 + 4);
@@ -536,29 +538,28 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
     }
     neg_count++;
     const int grp = neg_count % nbn;
-    if (lane == 0) {
-      int os = 1;
-      s.oidx[0] = t1;
-      s.ometa[0] = 1;
-      for (int k = 0; k < K; k++) {
-        const int v = s.negw[grp * K + k];
-        int hit = -1;
-        for (int i = 0; i < os; i++)
-          if (s.oidx[i] == v) { hit = i; break; }
-        if (hit < 0) {
-          s.oidx[os] = v; s.ometa[os] = 1; s.oslot[os] = s.negs[grp * K + k]; os++;
-        } else {
-          s.ometa[hit]++;
-        }
+    // Output rows (dsgl.hpp:1136-1158): the target, the distinct negatives of
+    // the group in first-draw order with their multiplicities (a negative
+    // equal to the target is dropped), then target2.  Lane k holds draw k.
+    int os;
+    {
+      const bool live = lane < K;
+      const int v = live ? s.negw[grp * K + lane] : -1;
+      const int vslot = live ? s.negs[grp * K + lane] : 0;
+      int first = lane, mult = 0;
+      for (int k2 = 0; k2 < K; k2++) {
+        const int w2 = __builtin_amdgcn_readlane(v, k2);
+        mult += (w2 == v) ? 1 : 0;
+        first = (w2 == v && k2 < first) ? k2 : first;
       }
-      s.ometa[0] = 1;  // a negative equal to the target is dropped
-      s.oidx[os] = t2;
-      s.ometa[os] = 1;
-      os++;
-      s.misc[0] = os;
+      const bool keep = live && first == lane && v != t1;
+      const u64 km = __ballot(keep);
+      const int o = 1 + __popcll(km & lanemask_lt(lane));
+      if (keep) { s.oidx[o] = v; s.ometa[o] = mult; s.oslot[o] = vslot; }
+      os = 2 + __popcll(km);
+      if (lane == 0) { s.oidx[0] = t1; s.ometa[0] = 1; s.oidx[os - 1] = t2; s.ometa[os - 1] = 1; }
     }
     __syncthreads();
-    const int os = s.misc[0];
     // Buffer row of each context, and the first context of the step that
     // shares it (repeat contexts reduce into that row, in context order).
     for (int j = lane; j < nin; j += 64) {
@@ -581,10 +582,11 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
     // Stage output rows (targets from Woh, negatives from the negative
     // buffer) and context rows (from the sentence buffers), B rows in flight.
     const int nrows = os + nin;
-    for (int r0 = 0; r0 < nrows; r0 += B) {
-      float2 v[B][NP];
+    constexpr int BS = (48 / NP) > 2 ? (48 / NP) : 2;  // a whole step's rows in one batch
+    for (int r0 = 0; r0 < nrows; r0 += BS) {
+      float2 v[BS][NP];
 #pragma unroll
-      for (int bb = 0; bb < B; bb++) {
+      for (int bb = 0; bb < BS; bb++) {
         const int r = r0 + bb;
         if (r < nrows) {
           const float* src = nullptr;
@@ -604,7 +606,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
         }
       }
 #pragma unroll
-      for (int bb = 0; bb < B; bb++) {
+      for (int bb = 0; bb < BS; bb++) {
         const int r = r0 + bb;
         if (r < nrows) {
           float* dst = (r < os) ? s.O + r * OST : s.I + (size_t)(r - os) * IST;
@@ -651,13 +653,28 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       const float2 v = ld2(x);
       st2(x, make_float2(v.x + g.x, v.y + g.y));
     });
-    for (int j = 0; j < nin; j++) {
-      if (s.crep[j] != j) continue;
-      float* dst = buf + (size_t)s.ckey[j] * D;
+    for (int j0 = 0; j0 < nin; j0 += 8) {
+      bool on[8];
+      int key[8];
+      float2 v[8][NP];
 #pragma unroll
-      for (int u = 0; u < NP; u++) {
-        const int pr = lane + 64 * u;
-        if (pr < DP) st2(dst + 2 * pr, ld2(s.I + (size_t)j * IST + 2 * pr));
+      for (int bb = 0; bb < 8; bb++) {
+        const int j = j0 + bb;
+        on[bb] = j < nin && s.crep[j] == j;
+        key[bb] = on[bb] ? s.ckey[j] : 0;
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          v[bb][u] = (on[bb] && pr < DP) ? ld2(s.I + (size_t)j * IST + 2 * pr) : make_float2(0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int bb = 0; bb < 8; bb++) {
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          if (on[bb] && pr < DP) st2(buf + (size_t)key[bb] * D + 2 * pr, v[bb][u]);
+        }
       }
     }
     steps++;

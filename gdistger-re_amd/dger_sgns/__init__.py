@@ -19,7 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdger_sgns.so")
+LIB_PATH = os.environ.get("DGER_SGNS_LIB") or os.path.join(_HERE, "libdger_sgns.so")
 
 DGER_OK = 0
 DGER_E_ARG = -1
