@@ -109,38 +109,40 @@ constexpr int kOC = 8;  // output rows per register chunk
 
 // LDS carve-up of one worker (all offsets 16-byte aligned).
 struct Lds {
-  int* tok[2];    // Lmax ranks per sentence (tok[1] == tok[0] + Lmax)
-  int* slot[2];   // Lmax: first position holding the same rank
-  int* negw;      // K*Lmax negative draws
-  int* negs;      // K*Lmax: first draw holding the same rank
-  int* cpos;      // 4W context positions
-  int* csent;     // 4W context sentence (0/1)
-  int* crep;      // 4W first context of the step with the same buffer row
-  int* ckey;      // 4W buffer row of the context: sentence*Lmax + slot
-  int* oidx;      // KO8 output ranks
-  int* ometa;     // KO8 multiplicities
-  int* oslot;     // KO8 negative-buffer rows
-  int* misc;      // 8 scalars
-  float* O;       // KO8 x OST staged output rows
-  float* I;       // 4W x IST staged context rows
-  float* Ct;      // 4W x CST gradient coefficients, transposed: Ct[j][i]
+  int* tok;     // 2 x Lmax ranks (sentence q at tok + q*Lmax)
+  int* slot;    // 2 x Lmax: first position of the same rank in the sentence
+  int* nxt;     // 2 x Lmax: next position of the same rank (kNone if none)
+  int* prv;     // 2 x Lmax: previous position of the same rank (-1 if none)
+  int* pent;    // 2 x Lmax: ring entry holding the row of each window position
+  int* negw;    // K*Lmax negative draws
+  int* negs;    // K*Lmax: first draw of the same rank
+  int* cpos;    // 4W context positions
+  int* csent;   // 4W context sentences
+  int* ckey;    // 4W ring rows of the contexts (q*RR + entry)
+  int* plans;   // two output plans (current step, next step), 3*KO8 ints each:
+                // ranks [0, KO8), multiplicities [KO8, 2*KO8), negative-buffer rows [2*KO8, 3*KO8)
+  float* ring;  // 2*RR rows x RST: the window rows of both sentences
+  float* O;     // KO8 x OST staged output rows
+  float* Ct;    // 4W x CST gradient coefficients, transposed: Ct[j][i]
 };
+
+constexpr int kNone = 1 << 30;
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 __host__ __device__ inline int ko8(int K) { return ((K + 2) + kOC - 1) & ~(kOC - 1); }
 __host__ __device__ inline int o_stride(int D) { return (D + 3) & ~3; }
-__host__ __device__ inline int i_stride(int D) { return ((D + 3) & ~3) + 4; }
+__host__ __device__ inline int ring_stride(int D) { return ((D + 3) & ~3) + 4; }
 __host__ __device__ inline int c_stride(int K) { return ko8(K); }
 
 __host__ __device__ inline size_t lds_layout(int Lmax, int K, int W, int D, size_t* off) {
-  const int KO = ko8(K), MC = 4 * W;
+  const int KO = ko8(K), MC = 4 * W, RR = 2 * W + 1;
   size_t o = 0;
-  off[0] = o; o = align16(o + 4 * (size_t)Lmax * 4);      // tok0, tok1, slot0, slot1
-  off[1] = o; o = align16(o + 2 * (size_t)K * Lmax * 4);  // negw, negs
-  off[2] = o; o = align16(o + 4 * (size_t)MC * 4);        // cpos, csent, crep, ckey
-  off[3] = o; o = align16(o + 3 * (size_t)KO * 4 + 8 * 4);// oidx, ometa, oslot, misc
-  off[4] = o; o = align16(o + (size_t)KO * o_stride(D) * 4);
-  off[5] = o; o = align16(o + (size_t)MC * i_stride(D) * 4);
+  off[0] = o; o = align16(o + 10 * (size_t)Lmax * 4);      // tok, slot, nxt, prv, pent (x2)
+  off[1] = o; o = align16(o + 2 * (size_t)K * Lmax * 4);   // negw, negs
+  off[2] = o; o = align16(o + 3 * (size_t)MC * 4);         // cpos, csent, ckey
+  off[3] = o; o = align16(o + 6 * (size_t)KO * 4);         // two output plans
+  off[4] = o; o = align16(o + 2 * (size_t)RR * ring_stride(D) * 4);
+  off[5] = o; o = align16(o + (size_t)KO * o_stride(D) * 4);
   off[6] = o; o = align16(o + (size_t)MC * c_stride(K) * 4);
   return o;
 }
@@ -151,18 +153,22 @@ __device__ inline Lds carve(unsigned char* smem, int Lmax, int K, int W, int D) 
   const int KO = ko8(K), MC = 4 * W;
   Lds s;
   int* b0 = reinterpret_cast<int*>(smem + off[0]);
-  s.tok[0] = b0; s.tok[1] = b0 + Lmax; s.slot[0] = b0 + 2 * Lmax; s.slot[1] = b0 + 3 * Lmax;
+  s.tok = b0; s.slot = b0 + 2 * Lmax; s.nxt = b0 + 4 * Lmax; s.prv = b0 + 6 * Lmax; s.pent = b0 + 8 * Lmax;
   int* b1 = reinterpret_cast<int*>(smem + off[1]);
   s.negw = b1; s.negs = b1 + K * Lmax;
   int* b2 = reinterpret_cast<int*>(smem + off[2]);
-  s.cpos = b2; s.csent = b2 + MC; s.crep = b2 + 2 * MC; s.ckey = b2 + 3 * MC;
+  s.cpos = b2; s.csent = b2 + MC; s.ckey = b2 + 2 * MC;
   int* b3 = reinterpret_cast<int*>(smem + off[3]);
-  s.oidx = b3; s.ometa = b3 + KO; s.oslot = b3 + 2 * KO; s.misc = b3 + 3 * KO;
-  s.O = reinterpret_cast<float*>(smem + off[4]);
-  s.I = reinterpret_cast<float*>(smem + off[5]);
+  s.plans = b3;
+  s.ring = reinterpret_cast<float*>(smem + off[4]);
+  s.O = reinterpret_cast<float*>(smem + off[5]);
   s.Ct = reinterpret_cast<float*>(smem + off[6]);
   return s;
 }
+
+__device__ __forceinline__ int* plan_idx(const Lds& s, int K, int pl) { return s.plans + pl * 3 * ko8(K); }
+__device__ __forceinline__ int* plan_meta(const Lds& s, int K, int pl) { return s.plans + pl * 3 * ko8(K) + ko8(K); }
+__device__ __forceinline__ int* plan_slot(const Lds& s, int K, int pl) { return s.plans + pl * 3 * ko8(K) + 2 * ko8(K); }
 
 // first[p] = smallest q <= p with v[q] == v[p], for p < n (values in LDS).
 // The inner loop has a wave-uniform trip count and broadcast reads.
@@ -181,41 +187,71 @@ __device__ inline void first_occurrence(const int* v, int* first, int n, int lan
   }
 }
 
-// GEMM1 + sigmoid (dsgl.hpp:1241-1272): lanes over contexts j; for each
-// chunk of 8 output rows, f[i] = sum_k O[i][k]*I[j][k] in sequential k with
-// one FMA per term.  Ct[j][i] = g * meta[i] for i < os, 0 for padding rows.
-__device__ inline void step_gemm1(const Lds& s, int D, int K, int os, int nin, int fin, float alpha,
-                                  const float* exp_table, int lane) {
-  const int OST = o_stride(D), IST = i_stride(D), CST = c_stride(K);
+// For each position p < n: first, previous and next position holding the
+// same value (kNone / -1 where there is none).
+__device__ inline void occurrences(const int* v, int* first, int* prev, int* next, int n, int lane) {
+  for (int c0 = 0; c0 < n; c0 += 64) {
+    const int p = c0 + lane;
+    const int x = (p < n) ? v[p] : 0;
+    int f = p, pv = -1, nx = kNone;
+#pragma unroll 8
+    for (int q = 0; q < n; q++) {
+      const bool eq = v[q] == x;
+      f = (eq && q < f) ? q : f;
+      pv = (eq && q < p) ? q : pv;
+      nx = (eq && q > p && nx == kNone) ? q : nx;
+    }
+    if (p < n) { first[p] = f; prev[p] = pv; next[p] = nx; }
+  }
+}
+
+// GEMM1 + sigmoid (dsgl.hpp:1241-1272): lanes over contexts j (row
+// rows + ckey[j]*RST); for each chunk of 8 output rows, f[i] = sum_k
+// O[i][k]*I[j][k] in sequential k with one FMA per term.  Ct[j][i] = g*meta[i]
+// for i < os, 0 for padding rows.
+__device__ inline void step_gemm1(const Lds& s, const float* rows, int RST, const int* ometa, int D, int K,
+                                  int os, int nin, int fin, float alpha, const float* exp_table, int lane) {
+  const int OST = o_stride(D), CST = c_stride(K);
   const int D4 = D >> 2;
   for (int j = lane; j < nin; j += 64) {
-    const float* x = s.I + (size_t)j * IST;
+    const float* x = rows + (size_t)s.ckey[j] * RST;
     for (int i0 = 0; i0 < os; i0 += kOC) {
       float acc[kOC];
 #pragma unroll
       for (int ii = 0; ii < kOC; ii++) acc[ii] = 0.f;
-#pragma unroll 2
+      // operands of k4+1 are read while k4 is multiplied
+      float4 xv = D4 > 0 ? *reinterpret_cast<const float4*>(x) : make_float4(0.f, 0.f, 0.f, 0.f);
+      float4 ov[kOC];
+#pragma unroll
+      for (int ii = 0; ii < kOC; ii++)
+        ov[ii] = D4 > 0 ? *reinterpret_cast<const float4*>(s.O + (i0 + ii) * OST) : make_float4(0.f, 0.f, 0.f, 0.f);
       for (int k4 = 0; k4 < D4; k4++) {
-        const float4 xv = *reinterpret_cast<const float4*>(x + 4 * k4);
+        const int kn = (k4 + 1 < D4) ? k4 + 1 : k4;
+        const float4 xn = *reinterpret_cast<const float4*>(x + 4 * kn);
+        float4 on[kOC];
+#pragma unroll
+        for (int ii = 0; ii < kOC; ii++) on[ii] = *reinterpret_cast<const float4*>(s.O + (i0 + ii) * OST + 4 * kn);
 #pragma unroll
         for (int ii = 0; ii < kOC; ii++) {
-          const float4 ov = *reinterpret_cast<const float4*>(s.O + (i0 + ii) * OST + 4 * k4);
-          acc[ii] = __builtin_fmaf(ov.x, xv.x, acc[ii]);
-          acc[ii] = __builtin_fmaf(ov.y, xv.y, acc[ii]);
-          acc[ii] = __builtin_fmaf(ov.z, xv.z, acc[ii]);
-          acc[ii] = __builtin_fmaf(ov.w, xv.w, acc[ii]);
+          acc[ii] = __builtin_fmaf(ov[ii].x, xv.x, acc[ii]);
+          acc[ii] = __builtin_fmaf(ov[ii].y, xv.y, acc[ii]);
+          acc[ii] = __builtin_fmaf(ov[ii].z, xv.z, acc[ii]);
+          acc[ii] = __builtin_fmaf(ov[ii].w, xv.w, acc[ii]);
         }
+        xv = xn;
+#pragma unroll
+        for (int ii = 0; ii < kOC; ii++) ov[ii] = on[ii];
       }
       for (int k = 4 * D4; k < D; k++) {
-        const float xv = x[k];
+        const float xk = x[k];
 #pragma unroll
-        for (int ii = 0; ii < kOC; ii++) acc[ii] = __builtin_fmaf(s.O[(i0 + ii) * OST + k], xv, acc[ii]);
+        for (int ii = 0; ii < kOC; ii++) acc[ii] = __builtin_fmaf(s.O[(i0 + ii) * OST + k], xk, acc[ii]);
       }
 #pragma unroll
       for (int ii = 0; ii < kOC; ii++) {
         const int i = i0 + ii;
         const int label = (j < fin) ? (i == 0 ? 1 : 0) : (i == os - 1 ? 1 : 0);
-        const float g = sig_grad(acc[ii], label, alpha, exp_table) * (float)s.ometa[i];
+        const float g = sig_grad(acc[ii], label, alpha, exp_table) * (float)ometa[i];
         s.Ct[(size_t)j * CST + i] = (i < os) ? g : 0.f;
       }
     }
@@ -225,28 +261,43 @@ __device__ inline void step_gemm1(const Lds& s, int D, int K, int os, int nin, i
 // GEMM2 (dsgl.hpp:1290): md[i] = sum_j C[i][j] * I[j][:] in sequential j;
 // lanes over element pairs, 8 output rows per chunk; sink(i, pr, md) for i < os.
 template <int NP, typename Sink>
-__device__ inline void step_gemm2(const Lds& s, int D, int K, int os, int nin, int lane, Sink&& sink) {
-  const int IST = i_stride(D), CST = c_stride(K), DP = D >> 1;
+__device__ inline void step_gemm2(const Lds& s, const float* rows, int RST, int D, int K, int os, int nin,
+                                  int lane, Sink&& sink) {
+  const int CST = c_stride(K), DP = D >> 1;
   for (int i0 = 0; i0 < os; i0 += kOC) {
     float2 acc[kOC][NP];
 #pragma unroll
     for (int ii = 0; ii < kOC; ii++)
 #pragma unroll
       for (int u = 0; u < NP; u++) acc[ii][u] = make_float2(0.f, 0.f);
-#pragma unroll 2
-    for (int j = 0; j < nin; j++) {
-      const float4 c0 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0);
-      const float4 c1 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0 + 4);
-      const float c[kOC] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    for (int j0 = 0; j0 < nin; j0 += 8) {
+      float4 cb[8][2];
+      float2 xb[8][NP];
 #pragma unroll
-      for (int u = 0; u < NP; u++) {
-        const int pr = lane + 64 * u;
-        const float2 x = (pr < DP) ? *reinterpret_cast<const float2*>(s.I + (size_t)j * IST + 2 * pr)
-                                   : make_float2(0.f, 0.f);
+      for (int b = 0; b < 8; b++) {
+        const int j = min(j0 + b, nin - 1);
+        cb[b][0] = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0);
+        cb[b][1] = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0 + 4);
+        const float* xr = rows + (size_t)s.ckey[j] * RST;
 #pragma unroll
-        for (int ii = 0; ii < kOC; ii++) {
-          acc[ii][u].x = __builtin_fmaf(c[ii], x.x, acc[ii][u].x);
-          acc[ii][u].y = __builtin_fmaf(c[ii], x.y, acc[ii][u].y);
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          xb[b][u] = (pr < DP) ? *reinterpret_cast<const float2*>(xr + 2 * pr) : make_float2(0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        if (j0 + b < nin) {
+          const float c[kOC] = {cb[b][0].x, cb[b][0].y, cb[b][0].z, cb[b][0].w,
+                                cb[b][1].x, cb[b][1].y, cb[b][1].z, cb[b][1].w};
+#pragma unroll
+          for (int u = 0; u < NP; u++) {
+#pragma unroll
+            for (int ii = 0; ii < kOC; ii++) {
+              acc[ii][u].x = __builtin_fmaf(c[ii], xb[b][u].x, acc[ii][u].x);
+              acc[ii][u].y = __builtin_fmaf(c[ii], xb[b][u].y, acc[ii][u].y);
+            }
+          }
         }
       }
     }
@@ -264,26 +315,31 @@ __device__ inline void step_gemm2(const Lds& s, int D, int K, int os, int nin, i
   }
 }
 
-// GEMM3 (dsgl.hpp:1309): gin[j] = sum_i C[i][j] * O[i][:] in sequential i;
-// lanes over element pairs; sink(j, pr, gin) for every context j in order.
-template <int NP, typename Sink>
-__device__ inline void step_gemm3(const Lds& s, int D, int K, int os, int nin, int lane, Sink&& sink) {
+// GEMM3 (dsgl.hpp:1309) for contexts j0..j0+7: g[b] = sum_i C[i][j0+b] *
+// O[i][:] in sequential i, for this lane's element pairs.  o0 holds output
+// rows 0..7; rows beyond 8 are read from LDS.
+template <int NP>
+__device__ inline void step_gemm3_batch(const Lds& s, int D, int K, int os, int nin, int j0, int lane,
+                                        const float2 (&o0)[kOC][NP], float2 (&g)[8][NP]) {
   const int OST = o_stride(D), CST = c_stride(K), DP = D >> 1;
-  float2 o0[kOC][NP];  // output rows 0..7 stay in registers
+  float4 cb[8][2];
 #pragma unroll
-  for (int ii = 0; ii < kOC; ii++)
+  for (int b = 0; b < 8; b++) {
+    const int j = min(j0 + b, nin - 1);
+    cb[b][0] = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST);
+    cb[b][1] = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + 4);
+  }
 #pragma unroll
-    for (int u = 0; u < NP; u++) {
-      const int pr = lane + 64 * u;
-      o0[ii][u] = (pr < DP) ? *reinterpret_cast<const float2*>(s.O + ii * OST + 2 * pr) : make_float2(0.f, 0.f);
-    }
-  for (int j = 0; j < nin; j++) {
-    float2 g[NP];
+  for (int b = 0; b < 8; b++) {
+    const int j = min(j0 + b, nin - 1);
 #pragma unroll
-    for (int u = 0; u < NP; u++) g[u] = make_float2(0.f, 0.f);
+    for (int u = 0; u < NP; u++) g[b][u] = make_float2(0.f, 0.f);
     for (int i0 = 0; i0 < os; i0 += kOC) {
-      const float4 c0 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0);
-      const float4 c1 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0 + 4);
+      float4 c0 = cb[b][0], c1 = cb[b][1];
+      if (i0 > 0) {
+        c0 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0);
+        c1 = *reinterpret_cast<const float4*>(s.Ct + (size_t)j * CST + i0 + 4);
+      }
       const float c[kOC] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
 #pragma unroll
       for (int ii = 0; ii < kOC; ii++) {
@@ -295,19 +351,26 @@ __device__ inline void step_gemm3(const Lds& s, int D, int K, int os, int nin, i
           if (i0 > 0)
             o = (pr < DP) ? *reinterpret_cast<const float2*>(s.O + (i0 + ii) * OST + 2 * pr)
                           : make_float2(0.f, 0.f);
-          const float gx = __builtin_fmaf(c[ii], o.x, g[u].x);
-          const float gy = __builtin_fmaf(c[ii], o.y, g[u].y);
-          g[u].x = on ? gx : g[u].x;
-          g[u].y = on ? gy : g[u].y;
+          const float gx = __builtin_fmaf(c[ii], o.x, g[b][u].x);
+          const float gy = __builtin_fmaf(c[ii], o.y, g[b][u].y);
+          g[b][u].x = on ? gx : g[b][u].x;
+          g[b][u].y = on ? gy : g[b][u].y;
         }
       }
     }
+  }
+}
+
+template <int NP>
+__device__ inline void load_o0(const Lds& s, int D, int lane, float2 (&o0)[kOC][NP]) {
+  const int OST = o_stride(D), DP = D >> 1;
+#pragma unroll
+  for (int ii = 0; ii < kOC; ii++)
 #pragma unroll
     for (int u = 0; u < NP; u++) {
       const int pr = lane + 64 * u;
-      if (pr < DP) sink(j, pr, g[u]);
+      o0[ii][u] = (pr < DP) ? *reinterpret_cast<const float2*>(s.O + ii * OST + 2 * pr) : make_float2(0.f, 0.f);
     }
-  }
 }
 
 __device__ __forceinline__ float2 ld2(const float* p) { return *reinterpret_cast<const float2*>(p); }
@@ -320,14 +383,13 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
   const int wk = blockIdx.x;
   const int D = a.D, W = a.W, K = a.K, Lmax = a.Lmax;
   const int DP = D >> 1;
-  const int MC = 4 * W, IST = i_stride(D), OST = o_stride(D);
+  const int MC = 4 * W, RR = 2 * W + 1, RST = ring_stride(D), OST = o_stride(D), KO = ko8(K);
   const Lds s = carve(smem, Lmax, K, W, D);
-  int* const tok = s.tok[0];    // sentence q: tok + q*Lmax
-  int* const slot = s.slot[0];  // sentence q: slot + q*Lmax
-  constexpr int B = (16 / NP) > 2 ? (16 / NP) : 2;  // rows with loads in flight per batch
+  constexpr int B = (32 / NP) > 2 ? (32 / NP) : 2;  // rows with loads in flight per batch
+  const bool can_plan = (K + 2 <= kOC);              // next-step outputs fit one register chunk
   // Output rows beyond the current step's count are read (and their results
   // discarded) by the 8-row GEMM chunks: start them finite.
-  for (int x = lane; x < ko8(K) * OST; x += 64) s.O[x] = 0.f;
+  for (int x = lane; x < KO * OST; x += 64) s.O[x] = 0.f;
   __syncthreads();
 
   // Corpus slice and seed of compute thread wk+1 (dsgl.hpp:791, :811-821).
@@ -344,25 +406,31 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
   u64 jA = 1, jC = 0;
   for (int i = 0; i <= lane; i++) { jA = jA * kLcgMul; jC = jC * kLcgMul + kLcgAdd; }
 
-  // Working buffers (dsgl.hpp:844-857): sentence q rows at buf + q*Lmax*D,
-  // indexed by the first position of each rank; negative rows at nbuf,
-  // indexed by the first draw of each rank.
+  // Working buffers (dsgl.hpp:844-857) in scratch memory: sentence q rows at
+  // buf + q*Lmax*D indexed by the first position of each rank (rows of ranks
+  // inside the current window live in the LDS ring instead); negative rows
+  // at nbuf indexed by the first draw of each rank.
   float* const buf = a.fscratch + (size_t)wk * worker_scratch_floats(Lmax, K, D);
   float* const nbuf = buf + 2 * (size_t)Lmax * D;
+  auto ring_row = [&](int q, int e) { return s.ring + (size_t)(q * RR + e) * RST; };
 
   const float a0 = a.alpha0;
   float alpha = a0;
   int iters_left = a.iter;
   u64 wc = 0, last_wc = 0, cur = start;
-  int len0 = 0, len1 = 0, pos0 = 0, pos1 = 0;
+  int len[2] = {0, 0}, pos[2] = {0, 0};
   int done_sent = 0, refresh = 1, neg_count = 0, nbn = 1;
   u64 steps = 0, raw = 0, sentences = 0;
   const int stride = record_stride(W, K);
   int32_t* const rec = a.rec ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
+  int plan = 0;           // which output plan the step uses
+  bool planned = false;   // the step's outputs were planned and prefetched by the previous step
+  int pre_os = 0;
+  float2 pre_o[kOC][NP];  // prefetched output rows of the next step
 
-  // Copies model rows src_rows[ids[k]] to dst + k*D for the k < n with
-  // first[k] == k (the first occurrence of each rank), B rows per batch.
-  auto gather = [&](const void* src_rows, const int* ids, const int* first, int n, float* dst) {
+  // Copies model rows src_rows[ids[k]] for the k < n with first[k] == k to
+  // dst(k), B rows per batch.
+  auto gather = [&](const void* src_rows, const int* ids, const int* first, int n, auto&& dst) {
     for (int k0 = 0; k0 < n; k0 += B) {
       float2 v[B][NP];
 #pragma unroll
@@ -380,10 +448,13 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       for (int b = 0; b < B; b++) {
         const int k = k0 + b;
         const bool on = k < n && first[k] == k;
+        if (on) {
+          float* d = dst(k);
 #pragma unroll
-        for (int u = 0; u < NP; u++) {
-          const int pr = lane + 64 * u;
-          if (on && pr < DP) st2(dst + (size_t)k * D + 2 * pr, v[b][u]);
+          for (int u = 0; u < NP; u++) {
+            const int pr = lane + 64 * u;
+            if (pr < DP) st2(d + 2 * pr, v[b][u]);
+          }
         }
       }
     }
@@ -416,6 +487,28 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       }
     }
   };
+  // Output list (dsgl.hpp:1136-1158) of a step with targets ta, tb and
+  // negative group g into plan pl: the target, the distinct negatives of the
+  // group in first-draw order with their multiplicities (a negative equal to
+  // the target is dropped), then tb.  Lane k holds draw k.  Returns the count.
+  auto make_outputs = [&](int pl, int ta, int tb, int g) -> int {
+    const bool live = lane < K;
+    const int v = live ? s.negw[g * K + lane] : -1;
+    const int vslot = live ? s.negs[g * K + lane] : 0;
+    int first = lane, mult = 0;
+    for (int k2 = 0; k2 < K; k2++) {
+      const int w2 = __builtin_amdgcn_readlane(v, k2);
+      mult += (w2 == v) ? 1 : 0;
+      first = (w2 == v && k2 < first) ? k2 : first;
+    }
+    const bool keep = live && first == lane && v != ta;
+    const u64 km = __ballot(keep);
+    const int o = 1 + __popcll(km & lanemask_lt(lane));
+    if (keep) { plan_idx(s, K, pl)[o] = v; plan_meta(s, K, pl)[o] = mult; plan_slot(s, K, pl)[o] = vslot; }
+    const int n = 2 + __popcll(km);
+    if (lane == 0) { plan_idx(s, K, pl)[0] = ta; plan_meta(s, K, pl)[0] = 1; plan_idx(s, K, pl)[n - 1] = tb; plan_meta(s, K, pl)[n - 1] = 1; }
+    return n;
+  };
 
   for (;;) {
     // Learning-rate decay in >10000-token chunks (dsgl.hpp:886-897).
@@ -430,11 +523,12 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
     }
 
     // Sentence loads (dsgl.hpp:902-965, :968-1030).
+#pragma unroll
     for (int q = 0; q < 2; q++) {
-      int* const tq = tok + q * Lmax;
-      int* const sq = slot + q * Lmax;
+      int* const tq = s.tok + q * Lmax;
+      int* const sq = s.slot + q * Lmax;
       for (;;) {
-        if ((q ? len1 : len0) != 0) break;
+        if (len[q] != 0) break;
         if (cur >= end) break;
         int L = 0;
         for (;;) {
@@ -475,12 +569,19 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
           wc += (u64)consumed;
           if (stop) break;
         }
-        if (q) { pos1 = 0; len1 = L > 1 ? L : 0; } else { pos0 = 0; len0 = L > 1 ? L : 0; }
+        pos[q] = 0;
+        len[q] = L > 1 ? L : 0;
         if (L <= 1) continue;
         __syncthreads();
-        first_occurrence(tq, sq, L, lane);
+        occurrences(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, lane);
         __syncthreads();
-        gather(a.wih, tq, sq, L, buf + (size_t)q * Lmax * D);
+        // Working rows of the sentence's distinct ranks: a rank first seen
+        // inside the first window [0, W] goes straight to ring entry = that
+        // position, the others to the scratch buffer.
+        float* const bq = buf + (size_t)q * Lmax * D;
+        gather(a.wih, tq, sq, L, [&](int k) { return k <= W ? ring_row(q, k) : bq + (size_t)k * D; });
+        for (int p = lane; p <= min(W, L - 1); p += 64) s.pent[q * Lmax + p] = sq[p];
+        planned = false;
       }
     }
 
@@ -490,130 +591,148 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       raw += wc;
       if (--iters_left == 0) break;
       wc = 0; cur = start; last_wc = 0;
-      len0 = 0; len1 = 0;
+      len[0] = 0; len[1] = 0;
+      planned = false;
       continue;
     }
 
     // Shared negative batch (dsgl.hpp:1054-1088).
     if (refresh) {
-      nbn = max(len0, len1);
+      nbn = max(len[0], len[1]);
       const int NN = nbn * K;
-      u64 s0 = rng;
-      for (int base = 0; base < NN; base += 64) {
-        const int c = min(64, NN - base);
-        if (lane < c) {
-          const u64 sr = jA * s0 + jC;
-          int v = a.table[(sr >> 16) % a.table_size];
-          if (!v) v = (int)(sr % (u64)(a.V - 1) + 1);
-          s.negw[base + lane] = v;
+      const u64 A64 = readlane64(jA, 63), C64 = readlane64(jC, 63);
+      const u64 r0 = rng;
+      u64 cs = r0;  // state before the first draw of the current 64-draw chunk
+      for (int base = 0; base < NN; base += 64 * 8) {
+        u64 sr[8];
+        int v[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          sr[c] = jA * cs + jC;
+          cs = A64 * cs + C64;
         }
-        s0 = readlane64(jA, c - 1) * s0 + readlane64(jC, c - 1);
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const int n = base + 64 * c + lane;
+          v[c] = (n < NN) ? a.table[(sr[c] >> 16) % a.table_size] : 1;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const int n = base + 64 * c + lane;
+          if (n < NN) s.negw[n] = v[c] ? v[c] : (int)(sr[c] % (u64)(a.V - 1) + 1);
+        }
       }
-      rng = s0;
+      {
+        u64 f = r0;
+        int left = NN;
+        while (left >= 64) { f = A64 * f + C64; left -= 64; }
+        if (left > 0) f = readlane64(jA, left - 1) * f + readlane64(jC, left - 1);
+        rng = f;
+      }
       __syncthreads();
       first_occurrence(s.negw, s.negs, NN, lane);
       __syncthreads();
-      gather(a.woh, s.negw, s.negs, NN, nbuf);
+      gather(a.woh, s.negw, s.negs, NN, [&](int k) { return nbuf + (size_t)k * D; });
       refresh = 0;
+      planned = false;
     }
 
     // ---- one step (dsgl.hpp:1089-1368) ----
-    const int t1 = tok[pos0];
-    const int t2 = tok[Lmax + pos1];
+    const int p0 = pos[0], p1 = pos[1];
+    const int t1 = s.tok[p0];
+    const int t2 = s.tok[Lmax + p1];
     rng = rng * kLcgMul + kLcgAdd;
     const int b = (int)(rng % (u64)W);
     const int hw = W - b;
-    const int lo0 = max(0, pos0 - hw), hi0 = min(len0 - 1, pos0 + hw);
-    const int lo1 = max(0, pos1 - hw), hi1 = min(len1 - 1, pos1 + hw);
-    const int n0 = (pos0 - lo0) + (hi0 - pos0);
-    const int n1 = (pos1 - lo1) + (hi1 - pos1);
+    const int lo0 = max(0, p0 - hw), hi0 = min(len[0] - 1, p0 + hw);
+    const int lo1 = max(0, p1 - hw), hi1 = min(len[1] - 1, p1 + hw);
+    const int n0 = (p0 - lo0) + (hi0 - p0);
+    const int n1 = (p1 - lo1) + (hi1 - p1);
     const int nin = n0 + n1, fin = n0;
     for (int j = lane; j < nin; j += 64) {
       const int q = (j < n0) ? 0 : 1;
       const int jj = q ? j - n0 : j;
-      const int lo = q ? lo1 : lo0, pp = q ? pos1 : pos0;
+      const int lo = q ? lo1 : lo0, pp = q ? p1 : p0;
       const int c = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
       s.cpos[j] = c;
       s.csent[j] = q;
+      s.ckey[j] = q * RR + s.pent[q * Lmax + c];
     }
     neg_count++;
     const int grp = neg_count % nbn;
-    // Output rows (dsgl.hpp:1136-1158): the target, the distinct negatives of
-    // the group in first-draw order with their multiplicities (a negative
-    // equal to the target is dropped), then target2.  Lane k holds draw k.
     int os;
-    {
-      const bool live = lane < K;
-      const int v = live ? s.negw[grp * K + lane] : -1;
-      const int vslot = live ? s.negs[grp * K + lane] : 0;
-      int first = lane, mult = 0;
-      for (int k2 = 0; k2 < K; k2++) {
-        const int w2 = __builtin_amdgcn_readlane(v, k2);
-        mult += (w2 == v) ? 1 : 0;
-        first = (w2 == v && k2 < first) ? k2 : first;
+    if (planned) {
+      os = pre_os;
+    } else {
+      os = make_outputs(plan, t1, t2, grp);
+    }
+    // Rows entering the windows at this step's transition, read now so the
+    // load overlaps the step (their scratch rows are final: the rank is not
+    // in the window).
+    float2 pin[2][NP];
+    bool pin_on[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int pq = q ? p1 : p0, Lq = len[q];
+      const int y = pq + W + 1, x = pq - W;
+      pin_on[q] = (pq + 1 < Lq) && (y < Lq) && (s.prv[q * Lmax + min(y, Lq - 1)] < max(x, 0));
+      const float* src = buf + ((size_t)q * Lmax + (size_t)s.slot[q * Lmax + min(y, Lq - 1)]) * D;
+#pragma unroll
+      for (int u = 0; u < NP; u++) {
+        const int pr = lane + 64 * u;
+        pin[q][u] = (pin_on[q] && pr < DP) ? ld2(src + 2 * pr) : make_float2(0.f, 0.f);
       }
-      const bool keep = live && first == lane && v != t1;
-      const u64 km = __ballot(keep);
-      const int o = 1 + __popcll(km & lanemask_lt(lane));
-      if (keep) { s.oidx[o] = v; s.ometa[o] = mult; s.oslot[o] = vslot; }
-      os = 2 + __popcll(km);
-      if (lane == 0) { s.oidx[0] = t1; s.ometa[0] = 1; s.oidx[os - 1] = t2; s.ometa[os - 1] = 1; }
     }
     __syncthreads();
-    // Buffer row of each context, and the first context of the step that
-    // shares it (repeat contexts reduce into that row, in context order).
-    for (int j = lane; j < nin; j += 64) {
-      const int q = s.csent[j];
-      s.ckey[j] = q * Lmax + slot[q * Lmax + s.cpos[j]];
-    }
-    __syncthreads();
+    // Repeated contexts share a ring row; the reduction path below keeps the
+    // reference's order for them.
+    bool has_dup = false;
     for (int j0 = 0; j0 < nin; j0 += 64) {
       const int j = j0 + lane;
       const int key = (j < nin) ? s.ckey[j] : -1;
-      const int lim = min(nin, j0 + 64);
-      int rep = j;
+      bool dup = false;
 #pragma unroll 8
-      for (int j2 = 0; j2 < lim; j2++) {
-        const int k2 = s.ckey[j2];
-        rep = (j2 < rep && k2 == key) ? j2 : rep;
-      }
-      if (j < nin) s.crep[j] = rep;
+      for (int j2 = 0; j2 < nin; j2++) dup = dup || (j2 < j && s.ckey[j2] == key);
+      has_dup = has_dup || (__ballot(j < nin && dup) != 0);
     }
-    // Stage output rows (targets from Woh, negatives from the negative
-    // buffer) and context rows (from the sentence buffers), B rows in flight.
-    const int nrows = os + nin;
-    constexpr int BS = (48 / NP) > 2 ? (48 / NP) : 2;  // a whole step's rows in one batch
-    for (int r0 = 0; r0 < nrows; r0 += BS) {
-      float2 v[BS][NP];
+    // Output rows: prefetched by the previous step, or loaded now.
+    if (planned) {
 #pragma unroll
-      for (int bb = 0; bb < BS; bb++) {
-        const int r = r0 + bb;
-        if (r < nrows) {
-          const float* src = nullptr;
-          size_t mrow = 0;
-          if (r < os) {
-            if (r == 0) mrow = (size_t)t1;
-            else if (r == os - 1) mrow = (size_t)t2;
-            else src = nbuf + (size_t)s.oslot[r] * D;
-          } else {
-            src = buf + (size_t)s.ckey[r - os] * D;
-          }
+      for (int i = 0; i < kOC; i++) {
+        if (i < os) {
 #pragma unroll
           for (int u = 0; u < NP; u++) {
             const int pr = lane + 64 * u;
-            if (pr < DP) v[bb][u] = src ? ld2(src + 2 * pr) : Row<R>::load(a.woh, mrow, D, pr);
+            if (pr < DP) st2(s.O + i * OST + 2 * pr, pre_o[i][u]);
           }
         }
       }
+    } else {
+      for (int r0 = 0; r0 < os; r0 += kOC) {
+        float2 v[kOC][NP];
 #pragma unroll
-      for (int bb = 0; bb < BS; bb++) {
-        const int r = r0 + bb;
-        if (r < nrows) {
-          float* dst = (r < os) ? s.O + r * OST : s.I + (size_t)(r - os) * IST;
+        for (int bb = 0; bb < kOC; bb++) {
+          const int r = r0 + bb;
+          if (r < os) {
 #pragma unroll
-          for (int u = 0; u < NP; u++) {
-            const int pr = lane + 64 * u;
-            if (pr < DP) st2(dst + 2 * pr, v[bb][u]);
+            for (int u = 0; u < NP; u++) {
+              const int pr = lane + 64 * u;
+              if (pr < DP)
+                v[bb][u] = (r == 0) ? Row<R>::load(a.woh, (size_t)t1, D, pr)
+                         : (r == os - 1) ? Row<R>::load(a.woh, (size_t)t2, D, pr)
+                         : ld2(nbuf + (size_t)plan_slot(s, K, plan)[r] * D + 2 * pr);
+            }
+          }
+        }
+#pragma unroll
+        for (int bb = 0; bb < kOC; bb++) {
+          const int r = r0 + bb;
+          if (r < os) {
+#pragma unroll
+            for (int u = 0; u < NP; u++) {
+              const int pr = lane + 64 * u;
+              if (pr < DP) st2(s.O + r * OST + 2 * pr, v[bb][u]);
+            }
           }
         }
       }
@@ -626,79 +745,192 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       r[0] = t1; r[1] = t2; r[2] = b; r[3] = fin; r[4] = nin; r[5] = os;
       r[6] = __float_as_int(alpha);
       r[7] = grp;
-      for (int j = 0; j < nin; j++) r[8 + j] = tok[s.csent[j] * Lmax + s.cpos[j]];
-      for (int i = 0; i < os; i++) { r[8 + MC + i] = s.oidx[i]; r[8 + MC + K + 2 + i] = s.ometa[i]; }
+      for (int j = 0; j < nin; j++) r[8 + j] = s.tok[s.csent[j] * Lmax + s.cpos[j]];
+      for (int i = 0; i < os; i++) {
+        r[8 + MC + i] = plan_idx(s, K, plan)[i];
+        r[8 + MC + K + 2 + i] = plan_meta(s, K, plan)[i];
+      }
     }
 
-    step_gemm1(s, D, K, os, nin, fin, alpha, a.exp_table, lane);
+    step_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, K, os, nin, fin, alpha, a.exp_table, lane);
     __syncthreads();
 
     // GEMM2 and the output-row scatter: target rows straight into Woh
     // (row 0 first, then row os-1, as dsgl.hpp:1352-1356), negatives into the
     // negative buffer (dsgl.hpp:1358-1366) from their staged values.
-    step_gemm2<NP>(s, D, K, os, nin, lane, [&](int i, int pr, float2 md) {
+    step_gemm2<NP>(s, s.ring, RST, D, K, os, nin, lane, [&](int i, int pr, float2 md) {
       if (i == 0) Row<R>::atomic_add(a.woh, (size_t)t1, D, pr, md);
       else if (i == os - 1) Row<R>::atomic_add(a.woh, (size_t)t2, D, pr, md);
       else {
         const float2 o = ld2(s.O + i * OST + 2 * pr);
-        st2(nbuf + (size_t)s.oslot[i] * D + 2 * pr, make_float2(o.x + md.x, o.y + md.y));
+        st2(nbuf + (size_t)plan_slot(s, K, plan)[i] * D + 2 * pr, make_float2(o.x + md.x, o.y + md.y));
       }
     });
-    // GEMM3 and the context scatter: updates reduced into the first context
-    // row of each buffer row in context order (dsgl.hpp:1314-1333), then one
-    // store per distinct buffer row.  Each lane owns its element pairs, so
-    // the reduction needs no barrier.
-    step_gemm3<NP>(s, D, K, os, nin, lane, [&](int j, int pr, float2 g) {
-      float* x = s.I + (size_t)s.crep[j] * IST + 2 * pr;
-      const float2 v = ld2(x);
-      st2(x, make_float2(v.x + g.x, v.y + g.y));
-    });
-    for (int j0 = 0; j0 < nin; j0 += 8) {
-      bool on[8];
-      int key[8];
-      float2 v[8][NP];
+
+    // Plan the next step's outputs and read its rows now (issued after this
+    // step's updates of the same rows, so they see them), when neither
+    // sentence ends here and no refresh can intervene.
+    const bool cont = can_plan && (p0 + 1 < len[0]) && (p1 + 1 < len[1]);
+    if (cont) {
+      const int np = plan ^ 1;
+      const int t1n = s.tok[p0 + 1], t2n = s.tok[Lmax + p1 + 1];
+      const int osn = make_outputs(np, t1n, t2n, (neg_count + 1) % nbn);
+      __syncthreads();
 #pragma unroll
-      for (int bb = 0; bb < 8; bb++) {
-        const int j = j0 + bb;
-        on[bb] = j < nin && s.crep[j] == j;
-        key[bb] = on[bb] ? s.ckey[j] : 0;
+      for (int i = 0; i < kOC; i++) {
+        if (i < osn) {
 #pragma unroll
-        for (int u = 0; u < NP; u++) {
-          const int pr = lane + 64 * u;
-          v[bb][u] = (on[bb] && pr < DP) ? ld2(s.I + (size_t)j * IST + 2 * pr) : make_float2(0.f, 0.f);
+          for (int u = 0; u < NP; u++) {
+            const int pr = lane + 64 * u;
+            if (pr < DP)
+              pre_o[i][u] = (i == 0) ? Row<R>::load(a.woh, (size_t)t1n, D, pr)
+                          : (i == osn - 1) ? Row<R>::load(a.woh, (size_t)t2n, D, pr)
+                          : ld2(nbuf + (size_t)plan_slot(s, K, np)[i] * D + 2 * pr);
+          }
         }
       }
+      pre_os = osn;
+    }
+
+    // GEMM3 and the context scatter (dsgl.hpp:1314-1333): each context's
+    // update is added into its ring row in context order, so repeated
+    // contexts accumulate as in the reference's sentence buffer.
+    {
+      float2 o0[kOC][NP];
+      load_o0<NP>(s, D, lane, o0);
+      for (int j0 = 0; j0 < nin; j0 += 8) {
+        float2 g[8][NP];
+        step_gemm3_batch<NP>(s, D, K, os, nin, j0, lane, o0, g);
+        if (!has_dup) {
+          float2 v[8][NP];
+          float* rp[8];
 #pragma unroll
-      for (int bb = 0; bb < 8; bb++) {
+          for (int bb = 0; bb < 8; bb++) {
+            const int j = min(j0 + bb, nin - 1);
+            rp[bb] = s.ring + (size_t)s.ckey[j] * RST;
 #pragma unroll
-        for (int u = 0; u < NP; u++) {
-          const int pr = lane + 64 * u;
-          if (on[bb] && pr < DP) st2(buf + (size_t)key[bb] * D + 2 * pr, v[bb][u]);
+            for (int u = 0; u < NP; u++) {
+              const int pr = lane + 64 * u;
+              v[bb][u] = (pr < DP) ? ld2(rp[bb] + 2 * pr) : make_float2(0.f, 0.f);
+            }
+          }
+#pragma unroll
+          for (int bb = 0; bb < 8; bb++) {
+            if (j0 + bb < nin) {
+#pragma unroll
+              for (int u = 0; u < NP; u++) {
+                const int pr = lane + 64 * u;
+                if (pr < DP) st2(rp[bb] + 2 * pr, make_float2(v[bb][u].x + g[bb][u].x, v[bb][u].y + g[bb][u].y));
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int bb = 0; bb < 8; bb++) {
+            const int j = j0 + bb;
+            if (j < nin) {
+              float* rr = s.ring + (size_t)s.ckey[j] * RST;
+#pragma unroll
+              for (int u = 0; u < NP; u++) {
+                const int pr = lane + 64 * u;
+                if (pr < DP) {
+                  const float2 x = ld2(rr + 2 * pr);
+                  st2(rr + 2 * pr, make_float2(x.x + g[bb][u].x, x.y + g[bb][u].y));
+                }
+              }
+            }
+          }
         }
       }
     }
     steps++;
+    __syncthreads();
 
-    // Write-backs (dsgl.hpp:1377-1407).
-    pos0++;
-    pos1++;
-    if (pos0 >= len0) {
-      scatter(a.wih, tok, slot, len0, buf);
-      len0 = 0;
-      done_sent++;
-      sentences++;
-    }
-    if (pos1 >= len1) {
-      scatter(a.wih, tok + Lmax, slot + Lmax, len1, buf + (size_t)Lmax * D);
-      len1 = 0;
-      done_sent++;
-      sentences++;
+    // Transition: advance both sentences; a finished sentence writes its
+    // buffer back over Wih (dsgl.hpp:1377-1395), a continuing one slides its
+    // window by one position.
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int p = pos[q];
+      const int Lq = len[q];
+      int* const tq = s.tok + q * Lmax;
+      int* const sq = s.slot + q * Lmax;
+      int* const pv = s.prv + q * Lmax;
+      int* const nx = s.nxt + q * Lmax;
+      int* const pe = s.pent + q * Lmax;
+      float* const bq = buf + (size_t)q * Lmax * D;
+      pos[q] = p + 1;
+      if (p + 1 >= Lq) {
+        // Rows still in the window go back to the scratch buffer, then every
+        // distinct rank's row overwrites its Wih row.
+        const int wlo = max(0, p - W);
+        for (int w = wlo; w < Lq; w++) {
+          if (pv[w] >= wlo) continue;
+          const float* src = ring_row(q, pe[w]);
+          float* dst = bq + (size_t)sq[w] * D;
+#pragma unroll
+          for (int u = 0; u < NP; u++) {
+            const int pr = lane + 64 * u;
+            if (pr < DP) st2(dst + 2 * pr, ld2(src + 2 * pr));
+          }
+        }
+        scatter(a.wih, tq, sq, Lq, bq);
+        len[q] = 0;
+        done_sent++;
+        sentences++;
+        continue;
+      }
+      const int x = p - W, y = p + W + 1;
+      if (x >= 0) {
+        // x is the first window position of its rank, so its row is in
+        // entry x mod RR; it moves to the rank's next window position, or
+        // back to the scratch buffer if the rank leaves the window.
+        const int e = x % RR;
+        const int z = nx[x];
+        if (z <= y) {
+          const int ez = z % RR;
+          if (ez != e) {
+#pragma unroll
+            for (int u = 0; u < NP; u++) {
+              const int pr = lane + 64 * u;
+              if (pr < DP) st2(ring_row(q, ez) + 2 * pr, ld2(ring_row(q, e) + 2 * pr));
+            }
+          }
+          const int sl = sq[x];
+          for (int w = x + 1 + lane; w <= min(y, Lq - 1); w += 64)
+            if (sq[w] == sl) pe[w] = ez;
+        } else {
+          float* dst = bq + (size_t)sq[x] * D;
+#pragma unroll
+          for (int u = 0; u < NP; u++) {
+            const int pr = lane + 64 * u;
+            if (pr < DP) st2(dst + 2 * pr, ld2(ring_row(q, e) + 2 * pr));
+          }
+        }
+      }
+      __syncthreads();
+      if (y < Lq) {
+        const int pvy = pv[y];
+        if (pvy >= max(x + 1, 0)) {
+          if (lane == 0) pe[y] = pe[pvy];
+        } else if (!(x >= 0 && pvy == x)) {
+          const int ey = y % RR;
+#pragma unroll
+          for (int u = 0; u < NP; u++) {
+            const int pr = lane + 64 * u;
+            if (pr < DP) st2(ring_row(q, ey) + 2 * pr, pin[q][u]);
+          }
+          if (lane == 0) pe[y] = ey;
+        }
+      }
     }
     if (done_sent >= 2) {
       scatter(a.woh, s.negw, s.negs, nbn * K, nbuf);
       refresh = 1;
       done_sent = 0;
     }
+    planned = cont;
+    if (cont) plan ^= 1;
     __syncthreads();
   }
 
@@ -775,22 +1007,37 @@ __global__ __launch_bounds__(64) void k_step(int D, int os, int nin, int fin, co
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x;
   const int K = os - 2, W = (nin + 3) / 4 > 0 ? (nin + 3) / 4 : 1;
-  const int OST = o_stride(D), IST = i_stride(D), CST = c_stride(K);
+  const int OST = o_stride(D), RST = ring_stride(D), CST = c_stride(K), DP = D >> 1;
   const Lds s = carve(smem, 1, K, W, D);
-  for (int i = lane; i < ko8(K); i += 64) s.ometa[i] = i < os ? meta[i] : 0;
+  for (int i = lane; i < ko8(K); i += 64) plan_meta(s, K, 0)[i] = i < os ? meta[i] : 0;
   for (int i = 0; i < ko8(K); i++)
     for (int d = lane; d < OST; d += 64) s.O[i * OST + d] = (i < os && d < D) ? O[(size_t)i * D + d] : 0.f;
   for (int j = 0; j < nin; j++)
-    for (int d = lane; d < D; d += 64) s.I[(size_t)j * IST + d] = I[(size_t)j * D + d];
+    for (int d = lane; d < D; d += 64) s.ring[(size_t)j * RST + d] = I[(size_t)j * D + d];
+  for (int j = lane; j < nin; j += 64) s.ckey[j] = j;
   __syncthreads();
-  step_gemm1(s, D, K, os, nin, fin, alpha, exp_table, lane);
+  step_gemm1(s, s.ring, RST, plan_meta(s, K, 0), D, K, os, nin, fin, alpha, exp_table, lane);
   __syncthreads();
   for (int i = 0; i < os; i++)
     for (int j = lane; j < nin; j += 64) corr[(size_t)i * nin + j] = s.Ct[(size_t)j * CST + i];
-  step_gemm2<NP>(s, D, K, os, nin, lane,
+  step_gemm2<NP>(s, s.ring, RST, D, K, os, nin, lane,
                  [&](int i, int pr, float2 v) { st2(md + (size_t)i * D + 2 * pr, v); });
-  step_gemm3<NP>(s, D, K, os, nin, lane,
-                 [&](int j, int pr, float2 v) { st2(gin + (size_t)j * D + 2 * pr, v); });
+  float2 o0[kOC][NP];
+  load_o0<NP>(s, D, lane, o0);
+  for (int j0 = 0; j0 < nin; j0 += 8) {
+    float2 g[8][NP];
+    step_gemm3_batch<NP>(s, D, K, os, nin, j0, lane, o0, g);
+#pragma unroll
+    for (int bb = 0; bb < 8; bb++) {
+      if (j0 + bb < nin) {
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          if (pr < DP) st2(gin + (size_t)(j0 + bb) * D + 2 * pr, g[bb][u]);
+        }
+      }
+    }
+  }
 }
 
 // ---- SGNS logistic loss and adjoint (exact sigmoid) -------------------------
