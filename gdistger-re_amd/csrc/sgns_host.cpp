@@ -124,7 +124,7 @@ std::vector<int64_t> table_starts(const std::vector<int32_t>& cn_rank, int64_t T
   return starts;
 }
 
-size_t row_bytes(int precision) { return precision == DGER_PRECISION_BF16 ? 2 : 4; }
+size_t row_bytes(int precision) { return precision == DGER_PRECISION_FP32 ? 4 : 2; }
 
 }  // namespace
 
@@ -237,6 +237,7 @@ int dger_sgns_parse_args(int argc, char** argv, dger_sgns_opts* o) {
   if ((v = val("-precision"))) {
     if (!*v) return missing("-precision");
     if (!std::strcmp(v, "bf16") || !std::strcmp(v, "1")) o->precision = DGER_PRECISION_BF16;
+    else if (!std::strcmp(v, "bf16-mfma") || !std::strcmp(v, "2")) o->precision = DGER_PRECISION_BF16_MFMA;
     else if (!std::strcmp(v, "fp32") || !std::strcmp(v, "0")) o->precision = DGER_PRECISION_FP32;
     else return fail(DGER_E_ARG, std::string("unknown -precision ") + v);
   }
@@ -257,7 +258,8 @@ static int check_opts(const dger_sgns_opts* o) {
   if (o->negative < 0 || o->negative > 62) return fail(DGER_E_UNSUPPORTED, "-negative must be in [0, 62]");
   if (o->iter < 1) return fail(DGER_E_ARG, "-iter must be >= 1");
   if (o->table_size < 1) return fail(DGER_E_ARG, "-table-size must be >= 1");
-  if (o->precision != DGER_PRECISION_FP32 && o->precision != DGER_PRECISION_BF16)
+  if (o->precision != DGER_PRECISION_FP32 && o->precision != DGER_PRECISION_BF16 &&
+      o->precision != DGER_PRECISION_BF16_MFMA)
     return fail(DGER_E_ARG, "unknown precision");
   if (o->workers < 0) return fail(DGER_E_ARG, "-workers must be >= 0");
   return DGER_OK;
@@ -266,7 +268,7 @@ static int check_opts(const dger_sgns_opts* o) {
 int dger_ctx_reset_model(dger_ctx* c) {
   if (!c) return fail(DGER_E_ARG, "null ctx");
   const uint64_t n = (uint64_t)c->V * (uint64_t)c->D;
-  if (c->o.precision == DGER_PRECISION_BF16) {
+  if (c->o.precision != DGER_PRECISION_FP32) {
     DGER_HIP(c->tmp_d.ensure(n * 4));
     DGER_HIP(dger::launch_init_net(c->tmp_d.as<float>(), n, c->D, c->stream));
     DGER_HIP(dger::launch_f32_to_bf16(c->tmp_d.as<float>(), c->wih_d.as<uint16_t>(), n, c->stream));
@@ -432,6 +434,7 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
   a.workers = workers;
   a.Lmax = c->Lmax;
   a.sample_on = c->o.sample > 0 ? 1 : 0;
+  a.mfma = c->o.precision == DGER_PRECISION_BF16_MFMA ? 1 : 0;
   a.alpha0 = c->o.alpha;
   a.alpha_den = (float)((uint64_t)c->o.iter * c->train_words + 1);
 
@@ -476,7 +479,7 @@ int dger_ctx_get_model(dger_ctx* c, float* wih, float* woh) {
   DevBuf* srcs[2] = {&c->wih_d, &c->woh_d};
   for (int m = 0; m < 2; m++) {
     if (!outs[m]) continue;
-    if (c->o.precision == DGER_PRECISION_BF16) {
+    if (c->o.precision != DGER_PRECISION_FP32) {
       DGER_HIP(c->tmp_d.ensure(n * 4));
       DGER_HIP(dger::launch_bf16_to_f32(srcs[m]->as<uint16_t>(), c->tmp_d.as<float>(), n, c->stream));
       DGER_HIP(hipMemcpyAsync(outs[m], c->tmp_d.p, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -496,7 +499,7 @@ int dger_ctx_set_model(dger_ctx* c, const float* wih, const float* woh) {
   DevBuf* dsts[2] = {&c->wih_d, &c->woh_d};
   for (int m = 0; m < 2; m++) {
     if (!ins[m]) continue;
-    if (c->o.precision == DGER_PRECISION_BF16) {
+    if (c->o.precision != DGER_PRECISION_FP32) {
       DGER_HIP(c->tmp_d.ensure(n * 4));
       DGER_HIP(hipMemcpyAsync(c->tmp_d.p, ins[m], n * 4, hipMemcpyHostToDevice, c->stream));
       DGER_HIP(dger::launch_f32_to_bf16(c->tmp_d.as<float>(), dsts[m]->as<uint16_t>(), n, c->stream));

@@ -39,6 +39,7 @@ struct TrainArgs {
   float* worker_alpha;       // per worker, final learning rate
   int32_t V, D, W, K, iter, workers, Lmax;
   int32_t sample_on;
+  int32_t mfma;              // BF16 rows: run the contractions on MFMA
   float alpha0;
   float alpha_den;           // (float)(iter * train_words + 1)
 };

@@ -23,6 +23,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dger_sgns.h"
 #include "sgns_internal.h"
 
@@ -133,6 +135,14 @@ __host__ __device__ inline int ko8(int K) { return ((K + 2) + kOC - 1) & ~(kOC -
 __host__ __device__ inline int o_stride(int D) { return (D + 3) & ~3; }
 __host__ __device__ inline int ring_stride(int D) { return ((D + 3) & ~3) + 4; }
 __host__ __device__ inline int c_stride(int K) { return ko8(K); }
+// MFMA path: coefficient tile Cji[j][i], 32*ceil(4W/32) context rows of
+// 16*ceil((K+2)/16) output columns.
+__host__ __device__ inline int cmf_stride(int K) { return ((K + 2 + 15) / 16) * 16; }
+__host__ __device__ inline int cmf_rows(int W) { return ((4 * W + 31) / 32) * 32; }
+__host__ __device__ inline size_t cmat_floats(int K, int W) {
+  const size_t fma = (size_t)(4 * W) * c_stride(K), mf = (size_t)cmf_rows(W) * cmf_stride(K);
+  return fma > mf ? fma : mf;
+}
 
 __host__ __device__ inline size_t lds_layout(int Lmax, int K, int W, int D, size_t* off) {
   const int KO = ko8(K), MC = 4 * W, RR = 2 * W + 1;
@@ -143,7 +153,7 @@ __host__ __device__ inline size_t lds_layout(int Lmax, int K, int W, int D, size
   off[3] = o; o = align16(o + 6 * (size_t)KO * 4);         // two output plans
   off[4] = o; o = align16(o + 2 * (size_t)RR * ring_stride(D) * 4);
   off[5] = o; o = align16(o + (size_t)KO * o_stride(D) * 4);
-  off[6] = o; o = align16(o + (size_t)MC * c_stride(K) * 4);
+  off[6] = o; o = align16(o + cmat_floats(K, W) * 4);
   return o;
 }
 
@@ -373,6 +383,187 @@ __device__ inline void load_o0(const Lds& s, int D, int lane, float2 (&o0)[kOC][
     }
 }
 
+// ---- BF16 form: the three contractions on MFMA --------------------------------
+// v_mfma_f32_16x16x32_bf16, lane l: A[l%16][8*(l/16)+e], B[8*(l/16)+e][l%16],
+// D[4*(l/16)+r][l%16].  Operands are the FP32 working rows rounded to bf16;
+// accumulation is FP32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 cvt8(float4 a, float4 b) {
+  bf16x8 r;
+  r[0] = (__bf16)a.x; r[1] = (__bf16)a.y; r[2] = (__bf16)a.z; r[3] = (__bf16)a.w;
+  r[4] = (__bf16)b.x; r[5] = (__bf16)b.y; r[6] = (__bf16)b.z; r[7] = (__bf16)b.w;
+  return r;
+}
+
+__device__ __forceinline__ bool mfma_ok(int D, int K, int W) {
+  return (D % 32) == 0 && D <= 256 && K + 2 <= 16 && W <= 16;
+}
+
+// GEMM1 + sigmoid: F[j][i] = sum_k I[j][k] O[i][k]; Cji[j][i] = g * meta[i]
+// (0 outside j < nin, i < os).  Writes 32*ceil(nin/32) context rows.
+__device__ inline void mfma_gemm1(const Lds& s, const float* rows, int RST, const int* ometa, int D, int K, int os,
+                                  int nin, int fin, float alpha, const float* exp_table, int lane) {
+  const int m = lane & 15, g = lane >> 4, OST = o_stride(D), CIS = cmf_stride(K);
+  const int njt = 2 * ((nin + 31) >> 5);
+  const int ir = m;  // one output tile (K + 2 <= 16)
+  const bool ion = ir < os;
+  const float* orow = s.O + (size_t)min(ir, os - 1) * OST;
+  const int nk = D >> 5;
+  for (int jt = 0; jt < njt; jt++) {
+    const int jr = min(16 * jt + m, nin - 1);
+    const float* xr = rows + (size_t)s.ckey[jr] * RST;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < nk; ks++) {
+      const int k0 = 32 * ks + 8 * g;
+      const float4 x0 = *reinterpret_cast<const float4*>(xr + k0);
+      const float4 x1 = *reinterpret_cast<const float4*>(xr + k0 + 4);
+      float4 o0 = *reinterpret_cast<const float4*>(orow + k0);
+      float4 o1 = *reinterpret_cast<const float4*>(orow + k0 + 4);
+      if (!ion) { o0 = make_float4(0.f, 0.f, 0.f, 0.f); o1 = o0; }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cvt8(x0, x1), cvt8(o0, o1), acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int j = 16 * jt + 4 * g + r;
+      const int i = m;
+      const int label = (j < fin) ? (i == 0 ? 1 : 0) : (i == os - 1 ? 1 : 0);
+      const float c = sig_grad(acc[r], label, alpha, exp_table) * (float)ometa[min(i, ko8(K) - 1)];
+      s.Ct[(size_t)j * CIS + i] = (j < nin && i < os) ? c : 0.f;
+    }
+  }
+}
+
+// ---- FP32 contractions with wave-uniform operands from registers ----------------
+// For steps with at most 64 contexts and 8 output rows: output rows and the
+// gradient coefficients are broadcast with v_readlane instead of LDS reads,
+// which leaves LDS bandwidth to the context rows.  Same sequential FMA order
+// as step_gemm1/2/3.
+__device__ __forceinline__ float rdl(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// GEMM1 + sigmoid: lane j < nin computes c[i] = g(f[i][j]) * meta[i] for the
+// 8 output rows, reading output rows as LDS broadcasts (operands of k4+1 are
+// read while k4 is multiplied).
+__device__ inline void fast_gemm1(const Lds& s, const float* ring, int RST, const int* ometa, int D, int os,
+                                  int nin, int fin, float alpha, const float* exp_table, int lane, int myk,
+                                  float (&c)[kOC]) {
+  const int OST = o_stride(D), D4 = D >> 2;
+  const float* x = ring + (size_t)myk * RST;
+  float acc[kOC];
+#pragma unroll
+  for (int ii = 0; ii < kOC; ii++) acc[ii] = 0.f;
+  if (D4 > 0) {
+    float4 xv = *reinterpret_cast<const float4*>(x);
+    float4 ov[kOC];
+#pragma unroll
+    for (int ii = 0; ii < kOC; ii++) ov[ii] = *reinterpret_cast<const float4*>(s.O + ii * OST);
+    for (int k4 = 0; k4 < D4; k4++) {
+      const int kn = (k4 + 1 < D4) ? k4 + 1 : k4;
+      const float4 xn = *reinterpret_cast<const float4*>(x + 4 * kn);
+      float4 on[kOC];
+#pragma unroll
+      for (int ii = 0; ii < kOC; ii++) on[ii] = *reinterpret_cast<const float4*>(s.O + ii * OST + 4 * kn);
+#pragma unroll
+      for (int ii = 0; ii < kOC; ii++) {
+        acc[ii] = __builtin_fmaf(ov[ii].x, xv.x, acc[ii]);
+        acc[ii] = __builtin_fmaf(ov[ii].y, xv.y, acc[ii]);
+        acc[ii] = __builtin_fmaf(ov[ii].z, xv.z, acc[ii]);
+        acc[ii] = __builtin_fmaf(ov[ii].w, xv.w, acc[ii]);
+      }
+      xv = xn;
+#pragma unroll
+      for (int ii = 0; ii < kOC; ii++) ov[ii] = on[ii];
+    }
+  }
+  for (int k = 4 * D4; k < D; k++) {
+    const float xk = x[k];
+#pragma unroll
+    for (int ii = 0; ii < kOC; ii++) acc[ii] = __builtin_fmaf(s.O[ii * OST + k], xk, acc[ii]);
+  }
+#pragma unroll
+  for (int ii = 0; ii < kOC; ii++) {
+    const int label = (lane < fin) ? (ii == 0 ? 1 : 0) : (ii == os - 1 ? 1 : 0);
+    const float g = sig_grad(acc[ii], label, alpha, exp_table) * (float)ometa[ii];
+    c[ii] = (ii < os && lane < nin) ? g : 0.f;
+  }
+}
+
+// GEMM2 (md[i] = sum_j c[i][j] I[j] in sequential j), and with `fuse` (no
+// repeated contexts) GEMM3 in the same pass: I[j] += sum_i c[i][j] O[i] in
+// sequential i, written back to the ring right after the row is read.
+template <int NP>
+__device__ inline void fast_gemm23(float* ring, int RST, int D, int os, int nin, int lane, int myk,
+                                   const float (&c)[kOC], const float2 (&o0)[kOC][NP], float2 (&md)[kOC][NP],
+                                   bool fuse) {
+  const int DP = D >> 1;
+#pragma unroll
+  for (int ii = 0; ii < kOC; ii++)
+#pragma unroll
+    for (int u = 0; u < NP; u++) md[ii][u] = make_float2(0.f, 0.f);
+  for (int j = 0; j < nin; j++) {
+    float cj[kOC];
+#pragma unroll
+    for (int ii = 0; ii < kOC; ii++) cj[ii] = rdl(c[ii], j);
+    float* xr = ring + (size_t)__builtin_amdgcn_readlane(myk, j) * RST;
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      const int pr = lane + 64 * u;
+      if (pr < DP) {
+        const float2 xv = *reinterpret_cast<const float2*>(xr + 2 * pr);
+#pragma unroll
+        for (int ii = 0; ii < kOC; ii++) {
+          md[ii][u].x = __builtin_fmaf(cj[ii], xv.x, md[ii][u].x);
+          md[ii][u].y = __builtin_fmaf(cj[ii], xv.y, md[ii][u].y);
+        }
+        if (fuse) {
+          float gx = 0.f, gy = 0.f;
+#pragma unroll
+          for (int ii = 0; ii < kOC; ii++) {
+            const float tx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
+            const float ty = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
+            gx = (ii < os) ? tx : gx;
+            gy = (ii < os) ? ty : gy;
+          }
+          *reinterpret_cast<float2*>(xr + 2 * pr) = make_float2(xv.x + gx, xv.y + gy);
+        }
+      }
+    }
+  }
+}
+
+// GEMM3 alone, for steps with repeated contexts: updates added into the ring
+// row of each context in context order.
+template <int NP>
+__device__ inline void fast_gemm3(float* ring, int RST, int D, int os, int nin, int lane, int myk,
+                                  const float (&c)[kOC], const float2 (&o0)[kOC][NP]) {
+  const int DP = D >> 1;
+  for (int j = 0; j < nin; j++) {
+    float cj[kOC];
+#pragma unroll
+    for (int ii = 0; ii < kOC; ii++) cj[ii] = rdl(c[ii], j);
+    float* xr = ring + (size_t)__builtin_amdgcn_readlane(myk, j) * RST;
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      const int pr = lane + 64 * u;
+      if (pr < DP) {
+        float gx = 0.f, gy = 0.f;
+#pragma unroll
+        for (int ii = 0; ii < kOC; ii++) {
+          const float tx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
+          const float ty = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
+          gx = (ii < os) ? tx : gx;
+          gy = (ii < os) ? ty : gy;
+        }
+        const float2 xv = *reinterpret_cast<const float2*>(xr + 2 * pr);
+        *reinterpret_cast<float2*>(xr + 2 * pr) = make_float2(xv.x + gx, xv.y + gy);
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ float2 ld2(const float* p) { return *reinterpret_cast<const float2*>(p); }
 __device__ __forceinline__ void st2(float* p, float2 v) { *reinterpret_cast<float2*>(p) = v; }
 
@@ -387,6 +578,8 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
   const Lds s = carve(smem, Lmax, K, W, D);
   constexpr int B = (32 / NP) > 2 ? (32 / NP) : 2;  // rows with loads in flight per batch
   const bool can_plan = (K + 2 <= kOC);              // next-step outputs fit one register chunk
+  // The BF16 form runs the three contractions on MFMA where the shapes allow.
+  const bool use_mfma = std::is_same<R, uint16_t>::value && a.mfma && mfma_ok(D, K, W);
   // Output rows beyond the current step's count are read (and their results
   // discarded) by the 8-row GEMM chunks: start them finite.
   for (int x = lane; x < KO * OST; x += 64) s.O[x] = 0.f;
@@ -752,12 +945,55 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       }
     }
 
-    step_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, K, os, nin, fin, alpha, a.exp_table, lane);
+    // Contractions: MFMA (BF16 form), register-broadcast FMA (small steps),
+    // or the LDS tile path (large windows / many negatives).
+    const bool fast = nin <= 64 && os <= kOC;
+    const int myk = s.ckey[min(lane, nin - 1)];
+    float cr[kOC];
+    float2 o0r[kOC][NP], mdr[kOC][NP];
+    if (fast) {
+      if (use_mfma) {
+        // GEMM1 on MFMA; lane j then takes its context's coefficients.
+        mfma_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, K, os, nin, fin, alpha, a.exp_table, lane);
+        __syncthreads();
+        const int CIS = cmf_stride(K);
+        const float4 c0 = *reinterpret_cast<const float4*>(s.Ct + (size_t)min(lane, nin - 1) * CIS);
+        const float4 c1 = *reinterpret_cast<const float4*>(s.Ct + (size_t)min(lane, nin - 1) * CIS + 4);
+        const float cc[kOC] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+        for (int ii = 0; ii < kOC; ii++) cr[ii] = (lane < nin) ? cc[ii] : 0.f;
+      } else {
+        fast_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, os, nin, fin, alpha, a.exp_table, lane, myk, cr);
+      }
+      load_o0<NP>(s, D, lane, o0r);
+      fast_gemm23<NP>(s.ring, RST, D, os, nin, lane, myk, cr, o0r, mdr, !has_dup);
+    } else {
+      step_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, K, os, nin, fin, alpha, a.exp_table, lane);
+    }
     __syncthreads();
 
     // GEMM2 and the output-row scatter: target rows straight into Woh
     // (row 0 first, then row os-1, as dsgl.hpp:1352-1356), negatives into the
     // negative buffer (dsgl.hpp:1358-1366) from their staged values.
+    if (fast) {
+      const int* oslot = plan_slot(s, K, plan);
+#pragma unroll
+      for (int ii = 0; ii < kOC; ii++) {
+        if (ii < os) {
+#pragma unroll
+          for (int u = 0; u < NP; u++) {
+            const int pr = lane + 64 * u;
+            if (pr < DP) {
+              if (ii == 0) Row<R>::atomic_add(a.woh, (size_t)t1, D, pr, mdr[0][u]);
+              else if (ii == os - 1) Row<R>::atomic_add(a.woh, (size_t)t2, D, pr, mdr[ii][u]);
+              else
+                st2(nbuf + (size_t)oslot[ii] * D + 2 * pr,
+                    make_float2(o0r[ii][u].x + mdr[ii][u].x, o0r[ii][u].y + mdr[ii][u].y));
+            }
+          }
+        }
+      }
+    } else
     step_gemm2<NP>(s, s.ring, RST, D, K, os, nin, lane, [&](int i, int pr, float2 md) {
       if (i == 0) Row<R>::atomic_add(a.woh, (size_t)t1, D, pr, md);
       else if (i == os - 1) Row<R>::atomic_add(a.woh, (size_t)t2, D, pr, md);
@@ -795,7 +1031,9 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
     // GEMM3 and the context scatter (dsgl.hpp:1314-1333): each context's
     // update is added into its ring row in context order, so repeated
     // contexts accumulate as in the reference's sentence buffer.
-    {
+    if (fast) {
+      if (has_dup) fast_gemm3<NP>(s.ring, RST, D, os, nin, lane, myk, cr, o0r);
+    } else {
       float2 o0[kOC][NP];
       load_o0<NP>(s, D, lane, o0);
       for (int j0 = 0; j0 < nin; j0 += 8) {
@@ -1154,7 +1392,8 @@ hipError_t launch_bf16_to_f32(const uint16_t* src, float* dst, uint64_t n, hipSt
 }
 
 hipError_t launch_train(const TrainArgs& a, int precision, hipStream_t s) {
-  if (precision == DGER_PRECISION_BF16) return launch_train_t<uint16_t>(a, s, precision);
+  if (precision == DGER_PRECISION_BF16 || precision == DGER_PRECISION_BF16_MFMA)
+    return launch_train_t<uint16_t>(a, s, precision);
   return launch_train_t<float>(a, s, precision);
 }
 

@@ -39,7 +39,8 @@ extern "C" {
 #define DGER_E_UNSUPPORTED (-8) /* option value outside what the kernels support */
 
 #define DGER_PRECISION_FP32 0   /* FP32 rows, FP32 FMA arithmetic */
-#define DGER_PRECISION_BF16 1   /* BF16 rows in HBM, FP32 working buffers */
+#define DGER_PRECISION_BF16 1   /* BF16 rows in HBM, FP32 working buffers, FP32 FMA contractions */
+#define DGER_PRECISION_BF16_MFMA 2 /* BF16 rows, contractions on MFMA (bf16 operands, FP32 accumulate) */
 
 /* Options.  The first block mirrors the flags dsgl() parses
  * (include/dsgl.hpp:1496-1535) with the same defaults (dsgl.hpp:67-76);
@@ -69,7 +70,7 @@ typedef struct dger_sgns_opts {
   int32_t workers;         /* -workers     concurrent Hogwild workers; 0 = auto.
                               Worker k trains the corpus slice and LCG seed of the
                               reference's compute thread k+1 under -threads workers+1. */
-  int32_t precision;       /* -precision   0 = fp32, 1 = bf16 */
+  int32_t precision;       /* -precision   fp32 (0), bf16 (1), bf16-mfma (2) */
   int64_t table_size;      /* -table-size  noise-table slots (1e8, dsgl.hpp:82) */
   int32_t device;          /* -device      HIP device ordinal (0) */
 } dger_sgns_opts;
