@@ -118,6 +118,7 @@ struct Lds {
   int* pent;    // 2 x Lmax: ring entry holding the row of each window position
   int* negw;    // K*Lmax negative draws
   int* negs;    // K*Lmax: first draw of the same rank
+  unsigned* nflag;  // bit per negative-buffer row: the row holds its snapshot
   int* cpos;    // 4W context positions
   int* csent;   // 4W context sentences
   int* ckey;    // 4W ring rows of the contexts (q*RR + entry)
@@ -135,6 +136,7 @@ __host__ __device__ inline int ko8(int K) { return ((K + 2) + kOC - 1) & ~(kOC -
 __host__ __device__ inline int o_stride(int D) { return (D + 3) & ~3; }
 __host__ __device__ inline int ring_stride(int D) { return ((D + 3) & ~3) + 4; }
 __host__ __device__ inline int c_stride(int K) { return ko8(K); }
+__host__ __device__ inline int nflag_words(int K, int Lmax) { return (K * Lmax + 31) / 32; }
 // MFMA path: coefficient tile Cji[j][i], 32*ceil(4W/32) context rows of
 // 16*ceil((K+2)/16) output columns.
 __host__ __device__ inline int cmf_stride(int K) { return ((K + 2 + 15) / 16) * 16; }
@@ -148,7 +150,7 @@ __host__ __device__ inline size_t lds_layout(int Lmax, int K, int W, int D, size
   const int KO = ko8(K), MC = 4 * W, RR = 2 * W + 1;
   size_t o = 0;
   off[0] = o; o = align16(o + 10 * (size_t)Lmax * 4);      // tok, slot, nxt, prv, pent (x2)
-  off[1] = o; o = align16(o + 2 * (size_t)K * Lmax * 4);   // negw, negs
+  off[1] = o; o = align16(o + (2 * (size_t)K * Lmax + nflag_words(K, Lmax)) * 4);  // negw, negs, nflag
   off[2] = o; o = align16(o + 3 * (size_t)MC * 4);         // cpos, csent, ckey
   off[3] = o; o = align16(o + 6 * (size_t)KO * 4);         // two output plans
   off[4] = o; o = align16(o + 2 * (size_t)RR * ring_stride(D) * 4);
@@ -166,6 +168,7 @@ __device__ inline Lds carve(unsigned char* smem, int Lmax, int K, int W, int D) 
   s.tok = b0; s.slot = b0 + 2 * Lmax; s.nxt = b0 + 4 * Lmax; s.prv = b0 + 6 * Lmax; s.pent = b0 + 8 * Lmax;
   int* b1 = reinterpret_cast<int*>(smem + off[1]);
   s.negw = b1; s.negs = b1 + K * Lmax;
+  s.nflag = reinterpret_cast<unsigned*>(b1 + 2 * K * Lmax);
   int* b2 = reinterpret_cast<int*>(smem + off[2]);
   s.cpos = b2; s.csent = b2 + MC; s.ckey = b2 + 2 * MC;
   int* b3 = reinterpret_cast<int*>(smem + off[3]);
@@ -654,13 +657,13 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
   };
   // The reverse copy: working rows back over the model rows (the memcpy
   // write-backs of dsgl.hpp:1377-1407).
-  auto scatter = [&](void* dst_rows, const int* ids, const int* first, int n, const float* src) {
+  auto scatter = [&](void* dst_rows, const int* ids, const int* first, int n, const float* src, auto&& keep) {
     for (int k0 = 0; k0 < n; k0 += B) {
       float2 v[B][NP];
 #pragma unroll
       for (int b = 0; b < B; b++) {
         const int k = k0 + b;
-        const bool on = k < n && first[k] == k;
+        const bool on = k < n && first[k] == k && keep(k);
 #pragma unroll
         for (int u = 0; u < NP; u++) {
           const int pr = lane + 64 * u;
@@ -670,7 +673,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
 #pragma unroll
       for (int b = 0; b < B; b++) {
         const int k = k0 + b;
-        const bool on = k < n && first[k] == k;
+        const bool on = k < n && first[k] == k && keep(k);
         const size_t row = on ? (size_t)ids[k] : 0;
 #pragma unroll
         for (int u = 0; u < NP; u++) {
@@ -701,6 +704,11 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
     const int n = 2 + __popcll(km);
     if (lane == 0) { plan_idx(s, K, pl)[0] = ta; plan_meta(s, K, pl)[0] = 1; plan_idx(s, K, pl)[n - 1] = tb; plan_meta(s, K, pl)[n - 1] = 1; }
     return n;
+  };
+
+  auto nsnap = [&](int sl) -> bool { return (s.nflag[sl >> 5] >> (sl & 31)) & 1u; };
+  auto nmark = [&](int sl) {
+    if (lane == 0) s.nflag[sl >> 5] |= 1u << (sl & 31);
   };
 
   for (;;) {
@@ -825,7 +833,12 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       __syncthreads();
       first_occurrence(s.negw, s.negs, NN, lane);
       __syncthreads();
-      gather(a.woh, s.negw, s.negs, NN, [&](int k) { return nbuf + (size_t)k * D; });
+      // The negative buffer holds a snapshot of each row taken at the refresh
+      // (dsgl.hpp:1081-1084).  Rows are snapshotted lazily: at first use as a
+      // negative (read from Woh then), or, if this worker is about to update
+      // the row as a target first, from that step's staged value.  Rows never
+      // snapshotted are unchanged by this worker and are not written back.
+      for (int w = lane; w < nflag_words(K, Lmax); w += 64) s.nflag[w] = 0u;
       refresh = 0;
       planned = false;
     }
@@ -913,7 +926,8 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
               if (pr < DP)
                 v[bb][u] = (r == 0) ? Row<R>::load(a.woh, (size_t)t1, D, pr)
                          : (r == os - 1) ? Row<R>::load(a.woh, (size_t)t2, D, pr)
-                         : ld2(nbuf + (size_t)plan_slot(s, K, plan)[r] * D + 2 * pr);
+                         : nsnap(plan_slot(s, K, plan)[r]) ? ld2(nbuf + (size_t)plan_slot(s, K, plan)[r] * D + 2 * pr)
+                                                            : Row<R>::load(a.woh, (size_t)plan_idx(s, K, plan)[r], D, pr);
             }
           }
         }
@@ -928,6 +942,29 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
             }
           }
         }
+      }
+    }
+    if (!planned)
+      for (int i = 1; i < os - 1; i++) nmark(plan_slot(s, K, plan)[i]);
+    // Targets that are pending negatives take their snapshot now, before this
+    // step's update of the row.
+#pragma unroll
+    for (int tq = 0; tq < 2; tq++) {
+      const int r = tq ? t2 : t1;
+      int sl = -1;
+      for (int c0 = 0; c0 < nbn * K && sl < 0; c0 += 64) {
+        const int n = c0 + lane;
+        const u64 hit = __ballot(n < nbn * K && s.negw[n] == r);
+        if (hit) sl = s.negs[c0 + __ffsll((long long)hit) - 1];
+      }
+      if (sl >= 0 && !nsnap(sl)) {
+        const float* src = s.O + (tq ? os - 1 : 0) * OST;
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+          const int pr = lane + 64 * u;
+          if (pr < DP) st2(nbuf + (size_t)sl * D + 2 * pr, ld2(src + 2 * pr));
+        }
+        nmark(sl);
       }
     }
     __syncthreads();
@@ -1021,11 +1058,13 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
             if (pr < DP)
               pre_o[i][u] = (i == 0) ? Row<R>::load(a.woh, (size_t)t1n, D, pr)
                           : (i == osn - 1) ? Row<R>::load(a.woh, (size_t)t2n, D, pr)
-                          : ld2(nbuf + (size_t)plan_slot(s, K, np)[i] * D + 2 * pr);
+                          : nsnap(plan_slot(s, K, np)[i]) ? ld2(nbuf + (size_t)plan_slot(s, K, np)[i] * D + 2 * pr)
+                                                           : Row<R>::load(a.woh, (size_t)plan_idx(s, K, np)[i], D, pr);
           }
         }
       }
       pre_os = osn;
+      for (int i = 1; i < osn - 1; i++) nmark(plan_slot(s, K, np)[i]);
     }
 
     // GEMM3 and the context scatter (dsgl.hpp:1314-1333): each context's
@@ -1112,7 +1151,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
             if (pr < DP) st2(dst + 2 * pr, ld2(src + 2 * pr));
           }
         }
-        scatter(a.wih, tq, sq, Lq, bq);
+        scatter(a.wih, tq, sq, Lq, bq, [](int) { return true; });
         len[q] = 0;
         done_sent++;
         sentences++;
@@ -1163,7 +1202,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       }
     }
     if (done_sent >= 2) {
-      scatter(a.woh, s.negw, s.negs, nbn * K, nbuf);
+      scatter(a.woh, s.negw, s.negs, nbn * K, nbuf, [&](int k) { return nsnap(k); });
       refresh = 1;
       done_sent = 0;
     }
