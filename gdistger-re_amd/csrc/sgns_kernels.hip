@@ -58,10 +58,16 @@ struct Row;
 
 template <>
 struct Row<float> {
-  __device__ static float2 load(const void* base, size_t row, int D, int pr) {
+  // Raw element pair as loaded; widen() converts it to two floats.
+  __device__ static u64 load_raw(const void* base, size_t row, int D, int pr) {
     const u64* p = reinterpret_cast<const u64*>(static_cast<const float*>(base) + row * (size_t)D) + pr;
-    u64 v = __hip_atomic_load(const_cast<u64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(const_cast<u64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static float2 widen(u64 v) {
     return make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
+  }
+  __device__ static float2 load(const void* base, size_t row, int D, int pr) {
+    return widen(load_raw(base, row, D, pr));
   }
   __device__ static void store(void* base, size_t row, int D, int pr, float2 v) {
     float2* p = reinterpret_cast<float2*>(static_cast<float*>(base) + row * (size_t)D) + pr;
@@ -76,11 +82,16 @@ struct Row<float> {
 
 template <>
 struct Row<uint16_t> {
-  __device__ static float2 load(const void* base, size_t row, int D, int pr) {
+  __device__ static u64 load_raw(const void* base, size_t row, int D, int pr) {
     const unsigned* p =
         reinterpret_cast<const unsigned*>(static_cast<const uint16_t*>(base) + row * (size_t)D) + pr;
-    unsigned v = __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return make_float2(bf16_to_f32(v & 0xFFFFu), bf16_to_f32(v >> 16));
+    return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static float2 widen(u64 v) {
+    return make_float2(bf16_to_f32((unsigned)v & 0xFFFFu), bf16_to_f32((unsigned)v >> 16));
+  }
+  __device__ static float2 load(const void* base, size_t row, int D, int pr) {
+    return widen(load_raw(base, row, D, pr));
   }
   __device__ static void store(void* base, size_t row, int D, int pr, float2 v) {
     unsigned* p = reinterpret_cast<unsigned*>(static_cast<uint16_t*>(base) + row * (size_t)D) + pr;
@@ -496,7 +507,8 @@ __device__ inline void fast_gemm1(const Lds& s, const float* ring, int RST, cons
 
 // GEMM2 (md[i] = sum_j c[i][j] I[j] in sequential j), and with `fuse` (no
 // repeated contexts) GEMM3 in the same pass: I[j] += sum_i c[i][j] O[i] in
-// sequential i, written back to the ring right after the row is read.
+// sequential i, written back to the ring right after the row is read.  The
+// row of context j+1 is read before row j is written (the rows differ).
 template <int NP>
 __device__ inline void fast_gemm23(float* ring, int RST, int D, int os, int nin, int lane, int myk,
                                    const float (&c)[kOC], const float2 (&o0)[kOC][NP], float2 (&md)[kOC][NP],
@@ -506,20 +518,35 @@ __device__ inline void fast_gemm23(float* ring, int RST, int D, int os, int nin,
   for (int ii = 0; ii < kOC; ii++)
 #pragma unroll
     for (int u = 0; u < NP; u++) md[ii][u] = make_float2(0.f, 0.f);
+  float2 xv[NP];
+  {
+    const float* xr = ring + (size_t)__builtin_amdgcn_readlane(myk, 0) * RST;
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      const int pr = lane + 64 * u;
+      xv[u] = (pr < DP) ? *reinterpret_cast<const float2*>(xr + 2 * pr) : make_float2(0.f, 0.f);
+    }
+  }
   for (int j = 0; j < nin; j++) {
     float cj[kOC];
 #pragma unroll
     for (int ii = 0; ii < kOC; ii++) cj[ii] = rdl(c[ii], j);
     float* xr = ring + (size_t)__builtin_amdgcn_readlane(myk, j) * RST;
+    const float* xn_r = ring + (size_t)__builtin_amdgcn_readlane(myk, min(j + 1, nin - 1)) * RST;
+    float2 xn[NP];
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      const int pr = lane + 64 * u;
+      xn[u] = (pr < DP) ? *reinterpret_cast<const float2*>(xn_r + 2 * pr) : make_float2(0.f, 0.f);
+    }
 #pragma unroll
     for (int u = 0; u < NP; u++) {
       const int pr = lane + 64 * u;
       if (pr < DP) {
-        const float2 xv = *reinterpret_cast<const float2*>(xr + 2 * pr);
 #pragma unroll
         for (int ii = 0; ii < kOC; ii++) {
-          md[ii][u].x = __builtin_fmaf(cj[ii], xv.x, md[ii][u].x);
-          md[ii][u].y = __builtin_fmaf(cj[ii], xv.y, md[ii][u].y);
+          md[ii][u].x = __builtin_fmaf(cj[ii], xv[u].x, md[ii][u].x);
+          md[ii][u].y = __builtin_fmaf(cj[ii], xv[u].y, md[ii][u].y);
         }
         if (fuse) {
           float gx = 0.f, gy = 0.f;
@@ -530,24 +557,46 @@ __device__ inline void fast_gemm23(float* ring, int RST, int D, int os, int nin,
             gx = (ii < os) ? tx : gx;
             gy = (ii < os) ? ty : gy;
           }
-          *reinterpret_cast<float2*>(xr + 2 * pr) = make_float2(xv.x + gx, xv.y + gy);
+          *reinterpret_cast<float2*>(xr + 2 * pr) = make_float2(xv[u].x + gx, xv[u].y + gy);
         }
       }
     }
+#pragma unroll
+    for (int u = 0; u < NP; u++) xv[u] = xn[u];
   }
 }
 
 // GEMM3 alone, for steps with repeated contexts: updates added into the ring
-// row of each context in context order.
+// row of each context in context order.  The row of context j+1 is read
+// before row j is written; when both are the same row the value just
+// computed is taken instead.
 template <int NP>
 __device__ inline void fast_gemm3(float* ring, int RST, int D, int os, int nin, int lane, int myk,
                                   const float (&c)[kOC], const float2 (&o0)[kOC][NP]) {
   const int DP = D >> 1;
+  float2 xv[NP];
+  {
+    const float* xr = ring + (size_t)__builtin_amdgcn_readlane(myk, 0) * RST;
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      const int pr = lane + 64 * u;
+      xv[u] = (pr < DP) ? *reinterpret_cast<const float2*>(xr + 2 * pr) : make_float2(0.f, 0.f);
+    }
+  }
   for (int j = 0; j < nin; j++) {
     float cj[kOC];
 #pragma unroll
     for (int ii = 0; ii < kOC; ii++) cj[ii] = rdl(c[ii], j);
-    float* xr = ring + (size_t)__builtin_amdgcn_readlane(myk, j) * RST;
+    const int kj = __builtin_amdgcn_readlane(myk, j);
+    const int kn = __builtin_amdgcn_readlane(myk, min(j + 1, nin - 1));
+    float* xr = ring + (size_t)kj * RST;
+    const float* xn_r = ring + (size_t)kn * RST;
+    float2 xn[NP];
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      const int pr = lane + 64 * u;
+      xn[u] = (pr < DP) ? *reinterpret_cast<const float2*>(xn_r + 2 * pr) : make_float2(0.f, 0.f);
+    }
 #pragma unroll
     for (int u = 0; u < NP; u++) {
       const int pr = lane + 64 * u;
@@ -560,10 +609,13 @@ __device__ inline void fast_gemm3(float* ring, int RST, int D, int os, int nin, 
           gx = (ii < os) ? tx : gx;
           gy = (ii < os) ? ty : gy;
         }
-        const float2 xv = *reinterpret_cast<const float2*>(xr + 2 * pr);
-        *reinterpret_cast<float2*>(xr + 2 * pr) = make_float2(xv.x + gx, xv.y + gy);
+        const float2 nv = make_float2(xv[u].x + gx, xv[u].y + gy);
+        *reinterpret_cast<float2*>(xr + 2 * pr) = nv;
+        if (kn == kj) xn[u] = nv;
       }
     }
+#pragma unroll
+    for (int u = 0; u < NP; u++) xv[u] = xn[u];
   }
 }
 
@@ -622,11 +674,14 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
   int plan = 0;           // which output plan the step uses
   bool planned = false;   // the step's outputs were planned and prefetched by the previous step
   int pre_os = 0;
-  float2 pre_o[kOC][NP];  // prefetched output rows of the next step
+  // Prefetched output rows of the next step, as loaded: FP32 pairs from the
+  // negative buffer where bit i of pre_nb is set, model-row pairs otherwise.
+  u64 pre_o[kOC][NP];
+  unsigned pre_nb = 0;
 
   // Copies model rows src_rows[ids[k]] for the k < n with first[k] == k to
-  // dst(k), B rows per batch.
-  auto gather = [&](const void* src_rows, const int* ids, const int* first, int n, auto&& dst) {
+  // ring entry k of sentence q (k <= W) or to row k of bq, B rows per batch.
+  auto gather = [&](const void* src_rows, const int* ids, const int* first, int n, int q, float* bq) {
     for (int k0 = 0; k0 < n; k0 += B) {
       float2 v[B][NP];
 #pragma unroll
@@ -640,12 +695,25 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
           if (on && pr < DP) v[b][u] = Row<R>::load(src_rows, row, D, pr);
         }
       }
+      // LDS and global destinations in separate loops, so that no store
+      // goes through a generic pointer (those also count against LDS waits).
 #pragma unroll
       for (int b = 0; b < B; b++) {
         const int k = k0 + b;
-        const bool on = k < n && first[k] == k;
-        if (on) {
-          float* d = dst(k);
+        if (k < n && k <= W && first[k] == k) {
+          float* d = ring_row(q, k);
+#pragma unroll
+          for (int u = 0; u < NP; u++) {
+            const int pr = lane + 64 * u;
+            if (pr < DP) st2(d + 2 * pr, v[b][u]);
+          }
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        const int k = k0 + b;
+        if (k < n && k > W && first[k] == k) {
+          float* d = bq + (size_t)k * D;
 #pragma unroll
           for (int u = 0; u < NP; u++) {
             const int pr = lane + 64 * u;
@@ -655,33 +723,60 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       }
     }
   };
-  // The reverse copy: working rows back over the model rows (the memcpy
-  // write-backs of dsgl.hpp:1377-1407).
-  auto scatter = [&](void* dst_rows, const int* ids, const int* first, int n, const float* src, auto&& keep) {
-    for (int k0 = 0; k0 < n; k0 += B) {
-      float2 v[B][NP];
+  // Write-backs (the memcpys of dsgl.hpp:1377-1407).  compact() lists the k
+  // in [c0, c1) with on(k) into lst (the staged-output area, free between
+  // steps) and returns the count; copy_back() writes the working rows
+  // src + lst[i]*D over model rows rank(lst[i]), B rows per batch, issuing
+  // the loads of batch i+1 before the stores of batch i.
+  int* const lst = reinterpret_cast<int*>(s.O);
+  const int lst_cap = KO * OST;
+  auto compact = [&](int c0, int c1, auto&& on) -> int {
+    int cnt = 0;
+    for (int k0 = c0; k0 < c1; k0 += 64) {
+      const int k = k0 + lane;
+      const bool x = k < c1 && on(k);
+      const u64 m = __ballot(x);
+      if (x) lst[cnt + __popcll(m & lanemask_lt(lane))] = k;
+      cnt += __popcll(m);
+    }
+    __syncthreads();
+    return cnt;
+  };
+  auto copy_back = [&](void* dst_rows, const float* src, int cnt, auto&& rank) {
+    float2 v[B][NP];
+    auto fetch = [&](int i0) {
 #pragma unroll
-      for (int b = 0; b < B; b++) {
-        const int k = k0 + b;
-        const bool on = k < n && first[k] == k && keep(k);
+      for (int bb = 0; bb < B; bb++) {
+        const int i = i0 + bb;
+        const int k = (i < cnt) ? lst[i] : 0;
 #pragma unroll
         for (int u = 0; u < NP; u++) {
           const int pr = lane + 64 * u;
-          if (on && pr < DP) v[b][u] = ld2(src + (size_t)k * D + 2 * pr);
+          if (i < cnt && pr < DP) v[bb][u] = ld2(src + (size_t)k * D + 2 * pr);
         }
       }
+    };
+    fetch(0);
+    for (int i0 = 0; i0 < cnt; i0 += B) {
+      float2 w[B][NP];
+      int rk[B];
 #pragma unroll
-      for (int b = 0; b < B; b++) {
-        const int k = k0 + b;
-        const bool on = k < n && first[k] == k && keep(k);
-        const size_t row = on ? (size_t)ids[k] : 0;
+      for (int bb = 0; bb < B; bb++) {
+        rk[bb] = (i0 + bb < cnt) ? rank(lst[i0 + bb]) : 0;
+#pragma unroll
+        for (int u = 0; u < NP; u++) w[bb][u] = v[bb][u];
+      }
+      if (i0 + B < cnt) fetch(i0 + B);
+#pragma unroll
+      for (int bb = 0; bb < B; bb++) {
 #pragma unroll
         for (int u = 0; u < NP; u++) {
           const int pr = lane + 64 * u;
-          if (on && pr < DP) Row<R>::store(dst_rows, row, D, pr, v[b][u]);
+          if (i0 + bb < cnt && pr < DP) Row<R>::store(dst_rows, (size_t)rk[bb], D, pr, w[bb][u]);
         }
       }
     }
+    __syncthreads();
   };
   // Output list (dsgl.hpp:1136-1158) of a step with targets ta, tb and
   // negative group g into plan pl: the target, the distinct negatives of the
@@ -780,7 +875,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
         // inside the first window [0, W] goes straight to ring entry = that
         // position, the others to the scratch buffer.
         float* const bq = buf + (size_t)q * Lmax * D;
-        gather(a.wih, tq, sq, L, [&](int k) { return k <= W ? ring_row(q, k) : bq + (size_t)k * D; });
+        gather(a.wih, tq, sq, L, q, bq);
         for (int p = lane; p <= min(W, L - 1); p += 64) s.pent[q * Lmax + p] = sq[p];
         planned = false;
       }
@@ -909,7 +1004,8 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
 #pragma unroll
           for (int u = 0; u < NP; u++) {
             const int pr = lane + 64 * u;
-            if (pr < DP) st2(s.O + i * OST + 2 * pr, pre_o[i][u]);
+            if (pr < DP)
+              st2(s.O + i * OST + 2 * pr, ((pre_nb >> i) & 1u) ? Row<float>::widen(pre_o[i][u]) : Row<R>::widen(pre_o[i][u]));
           }
         }
       }
@@ -1049,17 +1145,20 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       const int t1n = s.tok[p0 + 1], t2n = s.tok[Lmax + p1 + 1];
       const int osn = make_outputs(np, t1n, t2n, (neg_count + 1) % nbn);
       __syncthreads();
+      pre_nb = 0;
 #pragma unroll
       for (int i = 0; i < kOC; i++) {
         if (i < osn) {
+          const int sl = plan_slot(s, K, np)[i];
+          const bool from_nb = i > 0 && i < osn - 1 && nsnap(sl);
+          const int rk = (i == 0) ? t1n : (i == osn - 1) ? t2n : plan_idx(s, K, np)[i];
+          pre_nb |= from_nb ? (1u << i) : 0u;
 #pragma unroll
           for (int u = 0; u < NP; u++) {
             const int pr = lane + 64 * u;
             if (pr < DP)
-              pre_o[i][u] = (i == 0) ? Row<R>::load(a.woh, (size_t)t1n, D, pr)
-                          : (i == osn - 1) ? Row<R>::load(a.woh, (size_t)t2n, D, pr)
-                          : nsnap(plan_slot(s, K, np)[i]) ? ld2(nbuf + (size_t)plan_slot(s, K, np)[i] * D + 2 * pr)
-                                                           : Row<R>::load(a.woh, (size_t)plan_idx(s, K, np)[i], D, pr);
+              pre_o[i][u] = from_nb ? *reinterpret_cast<const u64*>(nbuf + (size_t)sl * D + 2 * pr)
+                                    : Row<R>::load_raw(a.woh, (size_t)rk, D, pr);
           }
         }
       }
@@ -1138,20 +1237,28 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       float* const bq = buf + (size_t)q * Lmax * D;
       pos[q] = p + 1;
       if (p + 1 >= Lq) {
-        // Rows still in the window go back to the scratch buffer, then every
-        // distinct rank's row overwrites its Wih row.
+        // Every distinct rank's working row overwrites its Wih row: ranks
+        // still in the window from their ring entry, the others from the
+        // scratch buffer (nx of a rank's first position marks the former; the
+        // links are not needed any more).
         const int wlo = max(0, p - W);
+        for (int w = wlo + lane; w < Lq; w += 64)
+          if (pv[w] < wlo) nx[sq[w]] = -2;
+        __syncthreads();
         for (int w = wlo; w < Lq; w++) {
           if (pv[w] >= wlo) continue;
           const float* src = ring_row(q, pe[w]);
-          float* dst = bq + (size_t)sq[w] * D;
+          const size_t row = (size_t)tq[w];
 #pragma unroll
           for (int u = 0; u < NP; u++) {
             const int pr = lane + 64 * u;
-            if (pr < DP) st2(dst + 2 * pr, ld2(src + 2 * pr));
+            if (pr < DP) Row<R>::store(a.wih, row, D, pr, ld2(src + 2 * pr));
           }
         }
-        scatter(a.wih, tq, sq, Lq, bq, [](int) { return true; });
+        for (int c0 = 0; c0 < Lq; c0 += lst_cap) {
+          const int cnt = compact(c0, min(Lq, c0 + lst_cap), [&](int k) { return sq[k] == k && nx[k] != -2; });
+          copy_back(a.wih, bq, cnt, [&](int k) { return tq[k]; });
+        }
         len[q] = 0;
         done_sent++;
         sentences++;
@@ -1202,7 +1309,12 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       }
     }
     if (done_sent >= 2) {
-      scatter(a.woh, s.negw, s.negs, nbn * K, nbuf, [&](int k) { return nsnap(k); });
+      // Rows that took a snapshot only.
+      const int NN = nbn * K;
+      for (int c0 = 0; c0 < NN; c0 += lst_cap) {
+        const int cnt = compact(c0, min(NN, c0 + lst_cap), [&](int k) { return s.negs[k] == k && nsnap(k); });
+        copy_back(a.woh, nbuf, cnt, [&](int k) { return s.negw[k]; });
+      }
       refresh = 1;
       done_sent = 0;
     }
