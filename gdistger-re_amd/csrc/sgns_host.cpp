@@ -373,13 +373,15 @@ int dger_ctx_set_corpus(dger_ctx* c, const int32_t* corpus, uint64_t n_tokens) {
   return DGER_OK;
 }
 
-static int auto_workers(dger_ctx* c, size_t lds) {
+// Default worker count: as many workgroups per CU as its LDS holds, at most
+// one wavefront per SIMD (the trainer uses more than 256 registers).
+static int auto_workers(dger_ctx* c, size_t block_lds, int wpb) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->o.device) != hipSuccess) return 1024;
   const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
                                                               : (size_t)160 * 1024;
-  int per_cu = (int)std::max<size_t>(1, lds_cu / std::max<size_t>(lds, 1));
-  per_cu = std::min(per_cu, 16);
+  int per_cu = (int)std::max<size_t>(1, lds_cu / std::max<size_t>(block_lds, 1)) * wpb;
+  per_cu = std::min(per_cu, 4);
   int w = prop.multiProcessorCount * per_cu;
   // Keep slices long enough that dropped in-flight sentences stay rare.
   const uint64_t max_by_corpus = std::max<uint64_t>(1, c->n_tokens / 4096);
@@ -393,14 +395,28 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
   if (!c->have_corpus) return fail(DGER_E_STATE, "dger_ctx_set_corpus must come first");
   if (rec && rec_cap <= 0) return fail(DGER_E_ARG, "rec_cap must be positive");
   DGER_HIP(hipSetDevice(c->o.device));
-  const size_t lds = dger::train_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision);
   hipDeviceProp_t prop;
   DGER_HIP(hipGetDeviceProperties(&prop, c->o.device));
-  if (lds > prop.sharedMemPerBlock)
-    return fail(DGER_E_UNSUPPORTED, "sentence length / window / negative need " + std::to_string(lds) +
-                                        " bytes of LDS, more than the device allows");
+  // Workgroups may take more LDS than the default per-block figure by opting in.
+  const size_t lds_block_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+  int wpb = dger::train_workers_per_block(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision,
+                                          lds_block_max);
+  if (wpb < 1)
+    return fail(DGER_E_UNSUPPORTED,
+                "sentence length / window / negative need " +
+                    std::to_string(dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D,
+                                                               c->o.precision, 1)) +
+                    " bytes of LDS, more than the device allows");
   if (workers <= 0) workers = c->o.workers;
-  if (workers <= 0) workers = auto_workers(c, lds);
+  if (workers <= 0)
+    workers = auto_workers(
+        c, dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision, wpb), wpb);
+  wpb = std::min(wpb, workers);
+  if (c->o.debug > 2)
+    fprintf(stderr, "dger: %d workers, %d per workgroup, LDS %zu bytes per workgroup (limit %zu, opt-in %zu)\n",
+            workers, wpb,
+            dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision, wpb),
+            (size_t)prop.sharedMemPerBlock, (size_t)prop.sharedMemPerBlockOptin);
   const size_t per = dger::worker_scratch_floats(c->Lmax, c->o.negative, c->D);
   const int stride = dger::record_stride(c->o.window, c->o.negative);
   if (c->scratch_d.ensure((size_t)workers * per * 4) || c->wsteps_d.ensure((size_t)workers * 8) ||
@@ -433,6 +449,7 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
   a.iter = c->o.iter;
   a.workers = workers;
   a.Lmax = c->Lmax;
+  a.wpb = wpb;
   a.sample_on = c->o.sample > 0 ? 1 : 0;
   a.mfma = c->o.precision == DGER_PRECISION_BF16_MFMA ? 1 : 0;
   a.alpha0 = c->o.alpha;

@@ -38,6 +38,7 @@ struct TrainArgs {
   unsigned long long* worker_steps;  // per worker
   float* worker_alpha;       // per worker, final learning rate
   int32_t V, D, W, K, iter, workers, Lmax;
+  int32_t wpb;  // workers (wavefronts) per workgroup
   int32_t sample_on;
   int32_t mfma;              // BF16 rows: run the contractions on MFMA
   float alpha0;
@@ -57,6 +58,10 @@ DGER_HD inline size_t worker_scratch_floats(int Lmax, int K, int D) {
 }
 
 size_t train_lds_bytes(int Lmax, int K, int W, int D, int precision);
+// Workers per workgroup that fit lds_per_block bytes (0 if not even one), at
+// most 4; and the workgroup's LDS bytes for wpb workers.
+int train_workers_per_block(int Lmax, int K, int W, int D, int precision, size_t lds_per_block);
+size_t train_block_lds_bytes(int Lmax, int K, int W, int D, int precision, int wpb);
 DGER_HD inline int record_stride(int W, int K) { return 8 + 4 * W + 2 * (K + 2); }
 
 hipError_t launch_init_net(float* wih, uint64_t n, int D, hipStream_t s);

@@ -110,12 +110,17 @@ struct Row<uint16_t> {
 // Table-quantised sigmoid gradient, dsgl.hpp:1249-1270.  The index is
 // clamped to the table only so that an undefined input can never read out of
 // bounds; for every finite f in [-6, 6] it is (int)((f + 6) * 83) <= 996.
+// The table is read unconditionally and the saturated cases selected after,
+// so that several lookups can be in flight at once.
 __device__ __forceinline__ float sig_grad(float f, int label, float alpha, const float* exp_table) {
-  if (f > 6.0f) return (float)(label - 1) * alpha;
-  if (f < -6.0f) return (float)label * alpha;
   int idx = (int)((f + 6.0f) * 83.0f);
   idx = min(max(idx, 0), kExpTable - 1);
-  return ((float)label - exp_table[idx]) * alpha;
+  const float e = exp_table[idx];
+  const float lab = (float)label;
+  const float mid = (lab - e) * alpha;
+  const float hi = (lab - 1.0f) * alpha;
+  const float lo = lab * alpha;
+  return f > 6.0f ? hi : (f < -6.0f ? lo : mid);
 }
 
 constexpr int kOC = 8;  // output rows per register chunk
@@ -141,6 +146,8 @@ struct Lds {
 };
 
 constexpr int kNone = 1 << 30;
+constexpr int kMaxWpb = 4;  // workers per workgroup: one wavefront per SIMD, the full register file each
+__host__ __device__ inline size_t exp_lds_bytes() { return ((size_t)kExpTable * 4 + 15) & ~(size_t)15; }
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 __host__ __device__ inline int ko8(int K) { return ((K + 2) + kOC - 1) & ~(kOC - 1); }
@@ -148,9 +155,9 @@ __host__ __device__ inline int o_stride(int D) { return (D + 3) & ~3; }
 __host__ __device__ inline int ring_stride(int D) { return ((D + 3) & ~3) + 4; }
 __host__ __device__ inline int c_stride(int K) { return ko8(K); }
 __host__ __device__ inline int nflag_words(int K, int Lmax) { return (K * Lmax + 31) / 32; }
-// MFMA path: coefficient tile Cji[j][i], 32*ceil(4W/32) context rows of
-// 16*ceil((K+2)/16) output columns.
-__host__ __device__ inline int cmf_stride(int K) { return ((K + 2 + 15) / 16) * 16; }
+// MFMA path (at most kOC output rows): coefficient tile Cji[j][i],
+// 32*ceil(4W/32) context rows of kOC output columns.
+__host__ __device__ inline int cmf_stride(int K) { (void)K; return kOC; }
 __host__ __device__ inline int cmf_rows(int W) { return ((4 * W + 31) / 32) * 32; }
 __host__ __device__ inline size_t cmat_floats(int K, int W) {
   const size_t fma = (size_t)(4 * W) * c_stride(K), mf = (size_t)cmf_rows(W) * cmf_stride(K);
@@ -416,35 +423,51 @@ __device__ __forceinline__ bool mfma_ok(int D, int K, int W) {
 }
 
 // GEMM1 + sigmoid: F[j][i] = sum_k I[j][k] O[i][k]; Cji[j][i] = g * meta[i]
-// (0 outside j < nin, i < os).  Writes 32*ceil(nin/32) context rows.
-__device__ inline void mfma_gemm1(const Lds& s, const float* rows, int RST, const int* ometa, int D, int K, int os,
+// (0 outside j < nin, i < os), for at most kOC output rows.  Two 16-row
+// context tiles per pass; columns i >= os take whatever the staged rows hold
+// and are discarded.  Writes 32*ceil(nin/32) context rows.
+template <int NP>
+__device__ inline void mfma_gemm1(const Lds& s, const float* rows, int RST, const int* ometa, int D, int os,
                                   int nin, int fin, float alpha, const float* exp_table, int lane) {
-  const int m = lane & 15, g = lane >> 4, OST = o_stride(D), CIS = cmf_stride(K);
-  const int njt = 2 * ((nin + 31) >> 5);
-  const int ir = m;  // one output tile (K + 2 <= 16)
-  const bool ion = ir < os;
-  const float* orow = s.O + (size_t)min(ir, os - 1) * OST;
+  const int m = lane & 15, g = lane >> 4, OST = o_stride(D), CIS = cmf_stride(0);
+  const int mi = min(m, kOC - 1);
+  const float* orow = static_cast<const float*>(__builtin_assume_aligned(s.O + (size_t)mi * OST, 16));
+  const float meta = (float)ometa[mi];
   const int nk = D >> 5;
-  for (int jt = 0; jt < njt; jt++) {
-    const int jr = min(16 * jt + m, nin - 1);
-    const float* xr = rows + (size_t)s.ckey[jr] * RST;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < nk; ks++) {
-      const int k0 = 32 * ks + 8 * g;
-      const float4 x0 = *reinterpret_cast<const float4*>(xr + k0);
-      const float4 x1 = *reinterpret_cast<const float4*>(xr + k0 + 4);
-      float4 o0 = *reinterpret_cast<const float4*>(orow + k0);
-      float4 o1 = *reinterpret_cast<const float4*>(orow + k0 + 4);
-      if (!ion) { o0 = make_float4(0.f, 0.f, 0.f, 0.f); o1 = o0; }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cvt8(x0, x1), cvt8(o0, o1), acc, 0, 0, 0);
-    }
+  for (int jt = 0; jt < 2 * ((nin + 31) >> 5); jt += 2) {
+    const float* xa = static_cast<const float*>(
+        __builtin_assume_aligned(rows + (size_t)s.ckey[min(16 * jt + m, nin - 1)] * RST, 16));
+    const float* xb = static_cast<const float*>(
+        __builtin_assume_aligned(rows + (size_t)s.ckey[min(16 * jt + 16 + m, nin - 1)] * RST, 16));
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int j = 16 * jt + 4 * g + r;
-      const int i = m;
-      const int label = (j < fin) ? (i == 0 ? 1 : 0) : (i == os - 1 ? 1 : 0);
-      const float c = sig_grad(acc[r], label, alpha, exp_table) * (float)ometa[min(i, ko8(K) - 1)];
-      s.Ct[(size_t)j * CIS + i] = (j < nin && i < os) ? c : 0.f;
+    for (int ks = 0; ks < 4 * NP; ks++) {
+      if (ks < nk) {
+        const int k0 = 32 * ks + 8 * g;
+        const float4 a0 = *reinterpret_cast<const float4*>(xa + k0);
+        const float4 a1 = *reinterpret_cast<const float4*>(xa + k0 + 4);
+        const float4 b0 = *reinterpret_cast<const float4*>(xb + k0);
+        const float4 b1 = *reinterpret_cast<const float4*>(xb + k0 + 4);
+        const float4 o0 = *reinterpret_cast<const float4*>(orow + k0);
+        const float4 o1 = *reinterpret_cast<const float4*>(orow + k0 + 4);
+        const bf16x8 ob = cvt8(o0, o1);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cvt8(a0, a1), ob, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cvt8(b0, b1), ob, acc1, 0, 0, 0);
+      }
+    }
+    float c[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int j = 16 * jt + 16 * (r >> 2) + 4 * g + (r & 3);
+      const int label = (j < fin) ? (m == 0 ? 1 : 0) : (m == os - 1 ? 1 : 0);
+      c[r] = sig_grad((r < 4) ? acc0[r & 3] : acc1[r & 3], label, alpha, exp_table) * meta;
+    }
+    if (m < kOC) {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const int j = 16 * jt + 16 * (r >> 2) + 4 * g + (r & 3);
+        s.Ct[(size_t)j * CIS + m] = (j < nin && m < os) ? c[r] : 0.f;
+      }
     }
   }
 }
@@ -549,13 +572,12 @@ __device__ inline void fast_gemm23(float* ring, int RST, int D, int os, int nin,
           md[ii][u].y = __builtin_fmaf(cj[ii], xv[u].y, md[ii][u].y);
         }
         if (fuse) {
+          // c[i] = 0 for i >= os: those terms add zero.
           float gx = 0.f, gy = 0.f;
 #pragma unroll
           for (int ii = 0; ii < kOC; ii++) {
-            const float tx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
-            const float ty = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
-            gx = (ii < os) ? tx : gx;
-            gy = (ii < os) ? ty : gy;
+            gx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
+            gy = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
           }
           *reinterpret_cast<float2*>(xr + 2 * pr) = make_float2(xv[u].x + gx, xv[u].y + gy);
         }
@@ -604,10 +626,8 @@ __device__ inline void fast_gemm3(float* ring, int RST, int D, int os, int nin, 
         float gx = 0.f, gy = 0.f;
 #pragma unroll
         for (int ii = 0; ii < kOC; ii++) {
-          const float tx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
-          const float ty = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
-          gx = (ii < os) ? tx : gx;
-          gy = (ii < os) ? ty : gy;
+          gx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
+          gy = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
         }
         const float2 nv = make_float2(xv[u].x + gx, xv[u].y + gy);
         *reinterpret_cast<float2*>(xr + 2 * pr) = nv;
@@ -622,15 +642,33 @@ __device__ inline void fast_gemm3(float* ring, int RST, int D, int os, int nin, 
 __device__ __forceinline__ float2 ld2(const float* p) { return *reinterpret_cast<const float2*>(p); }
 __device__ __forceinline__ void st2(float* p, float2 v) { *reinterpret_cast<float2*>(p) = v; }
 
+// Orders the accesses of one wavefront around a point of the step.  A
+// workgroup holds several independent workers (one per wavefront), so there
+// is no workgroup barrier inside the loop; the lanes of a wavefront observe
+// each other's LDS and memory accesses in program order, so this only keeps
+// the compiler from moving accesses across the point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One worker per wavefront, a.wpb workers per workgroup sharing the sigmoid
+// table in LDS; each worker's own LDS area follows the table.
 template <typename R, int NP>
-__global__ __launch_bounds__(64) void k_train(TrainArgs a) {
+__global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int lane = threadIdx.x;
-  const int wk = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wk = blockIdx.x * a.wpb + wave;
   const int D = a.D, W = a.W, K = a.K, Lmax = a.Lmax;
+  float* const expt = reinterpret_cast<float*>(smem);
+  for (int x = threadIdx.x; x < kExpTable; x += blockDim.x) expt[x] = a.exp_table[x];
+  __syncthreads();
+  if (wk >= a.workers) return;
   const int DP = D >> 1;
   const int MC = 4 * W, RR = 2 * W + 1, RST = ring_stride(D), OST = o_stride(D), KO = ko8(K);
-  const Lds s = carve(smem, Lmax, K, W, D);
+  size_t off_unused[7];
+  const Lds s = carve(smem + exp_lds_bytes() + (size_t)wave * lds_layout(Lmax, K, W, D, off_unused), Lmax, K, W, D);
   constexpr int B = (32 / NP) > 2 ? (32 / NP) : 2;  // rows with loads in flight per batch
   const bool can_plan = (K + 2 <= kOC);              // next-step outputs fit one register chunk
   // The BF16 form runs the three contractions on MFMA where the shapes allow.
@@ -638,7 +676,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
   // Output rows beyond the current step's count are read (and their results
   // discarded) by the 8-row GEMM chunks: start them finite.
   for (int x = lane; x < KO * OST; x += 64) s.O[x] = 0.f;
-  __syncthreads();
+  wave_sync();
 
   // Corpus slice and seed of compute thread wk+1 (dsgl.hpp:791, :811-821).
   const u64 N = a.n_tokens;
@@ -739,7 +777,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       if (x) lst[cnt + __popcll(m & lanemask_lt(lane))] = k;
       cnt += __popcll(m);
     }
-    __syncthreads();
+    wave_sync();
     return cnt;
   };
   auto copy_back = [&](void* dst_rows, const float* src, int cnt, auto&& rank) {
@@ -776,7 +814,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
         }
       }
     }
-    __syncthreads();
+    wave_sync();
   };
   // Output list (dsgl.hpp:1136-1158) of a step with targets ta, tb and
   // negative group g into plan pl: the target, the distinct negatives of the
@@ -868,9 +906,9 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
         pos[q] = 0;
         len[q] = L > 1 ? L : 0;
         if (L <= 1) continue;
-        __syncthreads();
+        wave_sync();
         occurrences(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, lane);
-        __syncthreads();
+        wave_sync();
         // Working rows of the sentence's distinct ranks: a rank first seen
         // inside the first window [0, W] goes straight to ring entry = that
         // position, the others to the scratch buffer.
@@ -925,9 +963,9 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
         if (left > 0) f = readlane64(jA, left - 1) * f + readlane64(jC, left - 1);
         rng = f;
       }
-      __syncthreads();
+      wave_sync();
       first_occurrence(s.negw, s.negs, NN, lane);
-      __syncthreads();
+      wave_sync();
       // The negative buffer holds a snapshot of each row taken at the refresh
       // (dsgl.hpp:1081-1084).  Rows are snapshotted lazily: at first use as a
       // negative (read from Woh then), or, if this worker is about to update
@@ -984,7 +1022,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
         pin[q][u] = (pin_on[q] && pr < DP) ? ld2(src + 2 * pr) : make_float2(0.f, 0.f);
       }
     }
-    __syncthreads();
+    wave_sync();
     // Repeated contexts share a ring row; the reduction path below keeps the
     // reference's order for them.
     bool has_dup = false;
@@ -1063,7 +1101,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
         nmark(sl);
       }
     }
-    __syncthreads();
+    wave_sync();
 
     if (rec && (int64_t)steps < a.rec_cap && lane == 0) {
       int32_t* r = rec + (size_t)steps * stride;
@@ -1087,8 +1125,8 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
     if (fast) {
       if (use_mfma) {
         // GEMM1 on MFMA; lane j then takes its context's coefficients.
-        mfma_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, K, os, nin, fin, alpha, a.exp_table, lane);
-        __syncthreads();
+        mfma_gemm1<NP>(s, s.ring, RST, plan_meta(s, K, plan), D, os, nin, fin, alpha, expt, lane);
+        wave_sync();
         const int CIS = cmf_stride(K);
         const float4 c0 = *reinterpret_cast<const float4*>(s.Ct + (size_t)min(lane, nin - 1) * CIS);
         const float4 c1 = *reinterpret_cast<const float4*>(s.Ct + (size_t)min(lane, nin - 1) * CIS + 4);
@@ -1096,14 +1134,14 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
 #pragma unroll
         for (int ii = 0; ii < kOC; ii++) cr[ii] = (lane < nin) ? cc[ii] : 0.f;
       } else {
-        fast_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, os, nin, fin, alpha, a.exp_table, lane, myk, cr);
+        fast_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, os, nin, fin, alpha, expt, lane, myk, cr);
       }
       load_o0<NP>(s, D, lane, o0r);
       fast_gemm23<NP>(s.ring, RST, D, os, nin, lane, myk, cr, o0r, mdr, !has_dup);
     } else {
-      step_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, K, os, nin, fin, alpha, a.exp_table, lane);
+      step_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, K, os, nin, fin, alpha, expt, lane);
     }
-    __syncthreads();
+    wave_sync();
 
     // GEMM2 and the output-row scatter: target rows straight into Woh
     // (row 0 first, then row os-1, as dsgl.hpp:1352-1356), negatives into the
@@ -1144,7 +1182,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       const int np = plan ^ 1;
       const int t1n = s.tok[p0 + 1], t2n = s.tok[Lmax + p1 + 1];
       const int osn = make_outputs(np, t1n, t2n, (neg_count + 1) % nbn);
-      __syncthreads();
+      wave_sync();
       pre_nb = 0;
 #pragma unroll
       for (int i = 0; i < kOC; i++) {
@@ -1220,7 +1258,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
       }
     }
     steps++;
-    __syncthreads();
+    wave_sync();
 
     // Transition: advance both sentences; a finished sentence writes its
     // buffer back over Wih (dsgl.hpp:1377-1395), a continuing one slides its
@@ -1244,7 +1282,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
         const int wlo = max(0, p - W);
         for (int w = wlo + lane; w < Lq; w += 64)
           if (pv[w] < wlo) nx[sq[w]] = -2;
-        __syncthreads();
+        wave_sync();
         for (int w = wlo; w < Lq; w++) {
           if (pv[w] >= wlo) continue;
           const float* src = ring_row(q, pe[w]);
@@ -1292,7 +1330,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
           }
         }
       }
-      __syncthreads();
+      wave_sync();
       if (y < Lq) {
         const int pvy = pv[y];
         if (pvy >= max(x + 1, 0)) {
@@ -1320,7 +1358,7 @@ __global__ __launch_bounds__(64) void k_train(TrainArgs a) {
     }
     planned = cont;
     if (cont) plan ^= 1;
-    __syncthreads();
+    wave_sync();
   }
 
   if (lane == 0) {
@@ -1496,14 +1534,19 @@ __global__ __launch_bounds__(64) void k_loss_bwd(const T* syn0, const T* syn1, i
 
 template <typename R>
 hipError_t launch_train_t(const TrainArgs& a, hipStream_t st, int precision) {
-  const size_t lds = train_lds_bytes(a.Lmax, a.K, a.W, a.D, precision);
+  if (a.wpb < 1 || a.wpb > kMaxWpb) return hipErrorInvalidValue;
+  const size_t lds = exp_lds_bytes() + (size_t)a.wpb * train_lds_bytes(a.Lmax, a.K, a.W, a.D, precision);
   const int np = (a.D / 2 + 63) / 64;
-  dim3 grid(a.workers), block(64);
+  dim3 grid((a.workers + a.wpb - 1) / a.wpb), block(64 * a.wpb);
   switch (np) {
-#define DGER_CASE(X)                                                      \
-  case X:                                                                 \
-    hipLaunchKernelGGL((k_train<R, X>), grid, block, lds, st, a);         \
-    break;
+#define DGER_CASE(X)                                                                              \
+  case X: {                                                                                       \
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train<R, X>),        \
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    if (e != hipSuccess) return e;                                                                \
+    hipLaunchKernelGGL((k_train<R, X>), grid, block, lds, st, a);                                 \
+    break;                                                                                        \
+  }
     DGER_CASE(1) DGER_CASE(2) DGER_CASE(3) DGER_CASE(4)
     DGER_CASE(5) DGER_CASE(6) DGER_CASE(7) DGER_CASE(8)
 #undef DGER_CASE
@@ -1518,6 +1561,17 @@ size_t train_lds_bytes(int Lmax, int K, int W, int D, int precision) {
   (void)precision;
   size_t off[7];
   return lds_layout(Lmax, K, W, D, off);
+}
+
+int train_workers_per_block(int Lmax, int K, int W, int D, int precision, size_t lds_per_block) {
+  const size_t per = train_lds_bytes(Lmax, K, W, D, precision);
+  if (lds_per_block < exp_lds_bytes() + per) return 0;
+  const size_t n = (lds_per_block - exp_lds_bytes()) / per;
+  return (int)(n < (size_t)kMaxWpb ? n : (size_t)kMaxWpb);
+}
+
+size_t train_block_lds_bytes(int Lmax, int K, int W, int D, int precision, int wpb) {
+  return exp_lds_bytes() + (size_t)wpb * train_lds_bytes(Lmax, K, W, D, precision);
 }
 
 hipError_t launch_init_net(float* wih, uint64_t n, int D, hipStream_t s) {
