@@ -485,8 +485,7 @@ __device__ __forceinline__ float rdl(float v, int l) {
 // 8 output rows, reading output rows as LDS broadcasts (operands of k4+1 are
 // read while k4 is multiplied).
 __device__ inline void fast_gemm1(const Lds& s, const float* ring, int RST, const int* ometa, int D, int os,
-                                  int nin, int fin, float alpha, const float* exp_table, int lane, int myk,
-                                  float (&c)[kOC]) {
+                                  int nin, int fin, float alpha, const float* exp_table, int lane, int myk) {
   const int OST = o_stride(D), D4 = D >> 2;
   const float* x = ring + (size_t)myk * RST;
   float acc[kOC];
@@ -520,71 +519,72 @@ __device__ inline void fast_gemm1(const Lds& s, const float* ring, int RST, cons
 #pragma unroll
     for (int ii = 0; ii < kOC; ii++) acc[ii] = __builtin_fmaf(s.O[ii * OST + k], xk, acc[ii]);
   }
+  float c[kOC];
 #pragma unroll
   for (int ii = 0; ii < kOC; ii++) {
     const int label = (lane < fin) ? (ii == 0 ? 1 : 0) : (ii == os - 1 ? 1 : 0);
     const float g = sig_grad(acc[ii], label, alpha, exp_table) * (float)ometa[ii];
-    c[ii] = (ii < os && lane < nin) ? g : 0.f;
+    c[ii] = (ii < os) ? g : 0.f;
+  }
+  if (lane < nin) {
+    float* ct = s.Ct + (size_t)lane * kOC;
+    *reinterpret_cast<float4*>(ct) = make_float4(c[0], c[1], c[2], c[3]);
+    *reinterpret_cast<float4*>(ct + 4) = make_float4(c[4], c[5], c[6], c[7]);
   }
 }
 
 // GEMM2 (md[i] = sum_j c[i][j] I[j] in sequential j), and with `fuse` (no
 // repeated contexts) GEMM3 in the same pass: I[j] += sum_i c[i][j] O[i] in
-// sequential i, written back to the ring right after the row is read.  The
-// row of context j+1 is read before row j is written (the rows differ).
+// sequential i, written back to the ring right after the row is read.
+// Coefficients come from Ct[j][0..kOC) (0 for i >= os, so those terms add
+// zero) as LDS broadcasts.  The row and coefficients of context j+1 are read
+// before row j is written (distinct rows).  Lanes past the row end compute on
+// whatever LDS holds there and store nothing.
 template <int NP>
-__device__ inline void fast_gemm23(float* ring, int RST, int D, int os, int nin, int lane, int myk,
-                                   const float (&c)[kOC], const float2 (&o0)[kOC][NP], float2 (&md)[kOC][NP],
-                                   bool fuse) {
+__device__ inline void fast_gemm23(float* ring, int RST, const float* Ct, int D, int nin, int lane, int myk,
+                                   const float2 (&o0)[kOC][NP], float2 (&md)[kOC][NP], bool fuse) {
   const int DP = D >> 1;
 #pragma unroll
   for (int ii = 0; ii < kOC; ii++)
 #pragma unroll
     for (int u = 0; u < NP; u++) md[ii][u] = make_float2(0.f, 0.f);
+  const float* xr0 = ring + (size_t)__builtin_amdgcn_readlane(myk, 0) * RST;
   float2 xv[NP];
-  {
-    const float* xr = ring + (size_t)__builtin_amdgcn_readlane(myk, 0) * RST;
 #pragma unroll
-    for (int u = 0; u < NP; u++) {
-      const int pr = lane + 64 * u;
-      xv[u] = (pr < DP) ? *reinterpret_cast<const float2*>(xr + 2 * pr) : make_float2(0.f, 0.f);
-    }
-  }
+  for (int u = 0; u < NP; u++) xv[u] = *reinterpret_cast<const float2*>(xr0 + 2 * (lane + 64 * u));
+  float4 ca = *reinterpret_cast<const float4*>(Ct), cb = *reinterpret_cast<const float4*>(Ct + 4);
   for (int j = 0; j < nin; j++) {
-    float cj[kOC];
-#pragma unroll
-    for (int ii = 0; ii < kOC; ii++) cj[ii] = rdl(c[ii], j);
+    const int jn = min(j + 1, nin - 1);
     float* xr = ring + (size_t)__builtin_amdgcn_readlane(myk, j) * RST;
-    const float* xn_r = ring + (size_t)__builtin_amdgcn_readlane(myk, min(j + 1, nin - 1)) * RST;
+    const float* xn_r = ring + (size_t)__builtin_amdgcn_readlane(myk, jn) * RST;
     float2 xn[NP];
 #pragma unroll
-    for (int u = 0; u < NP; u++) {
-      const int pr = lane + 64 * u;
-      xn[u] = (pr < DP) ? *reinterpret_cast<const float2*>(xn_r + 2 * pr) : make_float2(0.f, 0.f);
-    }
+    for (int u = 0; u < NP; u++) xn[u] = *reinterpret_cast<const float2*>(xn_r + 2 * (lane + 64 * u));
+    const float4 na = *reinterpret_cast<const float4*>(Ct + (size_t)jn * kOC);
+    const float4 nb = *reinterpret_cast<const float4*>(Ct + (size_t)jn * kOC + 4);
+    const float cj[kOC] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
 #pragma unroll
     for (int u = 0; u < NP; u++) {
       const int pr = lane + 64 * u;
-      if (pr < DP) {
+#pragma unroll
+      for (int ii = 0; ii < kOC; ii++) {
+        md[ii][u].x = __builtin_fmaf(cj[ii], xv[u].x, md[ii][u].x);
+        md[ii][u].y = __builtin_fmaf(cj[ii], xv[u].y, md[ii][u].y);
+      }
+      if (fuse) {
+        float gx = 0.f, gy = 0.f;
 #pragma unroll
         for (int ii = 0; ii < kOC; ii++) {
-          md[ii][u].x = __builtin_fmaf(cj[ii], xv[u].x, md[ii][u].x);
-          md[ii][u].y = __builtin_fmaf(cj[ii], xv[u].y, md[ii][u].y);
+          gx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
+          gy = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
         }
-        if (fuse) {
-          // c[i] = 0 for i >= os: those terms add zero.
-          float gx = 0.f, gy = 0.f;
-#pragma unroll
-          for (int ii = 0; ii < kOC; ii++) {
-            gx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
-            gy = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
-          }
-          *reinterpret_cast<float2*>(xr + 2 * pr) = make_float2(xv[u].x + gx, xv[u].y + gy);
-        }
+        if (pr < DP) *reinterpret_cast<float2*>(xr + 2 * pr) = make_float2(xv[u].x + gx, xv[u].y + gy);
       }
     }
 #pragma unroll
     for (int u = 0; u < NP; u++) xv[u] = xn[u];
+    ca = na;
+    cb = nb;
   }
 }
 
@@ -593,49 +593,43 @@ __device__ inline void fast_gemm23(float* ring, int RST, int D, int os, int nin,
 // before row j is written; when both are the same row the value just
 // computed is taken instead.
 template <int NP>
-__device__ inline void fast_gemm3(float* ring, int RST, int D, int os, int nin, int lane, int myk,
-                                  const float (&c)[kOC], const float2 (&o0)[kOC][NP]) {
+__device__ inline void fast_gemm3(float* ring, int RST, const float* Ct, int D, int nin, int lane, int myk,
+                                  const float2 (&o0)[kOC][NP]) {
   const int DP = D >> 1;
+  const float* xr0 = ring + (size_t)__builtin_amdgcn_readlane(myk, 0) * RST;
   float2 xv[NP];
-  {
-    const float* xr = ring + (size_t)__builtin_amdgcn_readlane(myk, 0) * RST;
 #pragma unroll
-    for (int u = 0; u < NP; u++) {
-      const int pr = lane + 64 * u;
-      xv[u] = (pr < DP) ? *reinterpret_cast<const float2*>(xr + 2 * pr) : make_float2(0.f, 0.f);
-    }
-  }
+  for (int u = 0; u < NP; u++) xv[u] = *reinterpret_cast<const float2*>(xr0 + 2 * (lane + 64 * u));
+  float4 ca = *reinterpret_cast<const float4*>(Ct), cb = *reinterpret_cast<const float4*>(Ct + 4);
   for (int j = 0; j < nin; j++) {
-    float cj[kOC];
-#pragma unroll
-    for (int ii = 0; ii < kOC; ii++) cj[ii] = rdl(c[ii], j);
+    const int jn = min(j + 1, nin - 1);
     const int kj = __builtin_amdgcn_readlane(myk, j);
-    const int kn = __builtin_amdgcn_readlane(myk, min(j + 1, nin - 1));
+    const int kn = __builtin_amdgcn_readlane(myk, jn);
     float* xr = ring + (size_t)kj * RST;
     const float* xn_r = ring + (size_t)kn * RST;
     float2 xn[NP];
 #pragma unroll
-    for (int u = 0; u < NP; u++) {
-      const int pr = lane + 64 * u;
-      xn[u] = (pr < DP) ? *reinterpret_cast<const float2*>(xn_r + 2 * pr) : make_float2(0.f, 0.f);
-    }
+    for (int u = 0; u < NP; u++) xn[u] = *reinterpret_cast<const float2*>(xn_r + 2 * (lane + 64 * u));
+    const float4 na = *reinterpret_cast<const float4*>(Ct + (size_t)jn * kOC);
+    const float4 nb = *reinterpret_cast<const float4*>(Ct + (size_t)jn * kOC + 4);
+    const float cj[kOC] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
 #pragma unroll
     for (int u = 0; u < NP; u++) {
       const int pr = lane + 64 * u;
-      if (pr < DP) {
-        float gx = 0.f, gy = 0.f;
+      float gx = 0.f, gy = 0.f;
 #pragma unroll
-        for (int ii = 0; ii < kOC; ii++) {
-          gx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
-          gy = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
-        }
-        const float2 nv = make_float2(xv[u].x + gx, xv[u].y + gy);
-        *reinterpret_cast<float2*>(xr + 2 * pr) = nv;
-        if (kn == kj) xn[u] = nv;
+      for (int ii = 0; ii < kOC; ii++) {
+        gx = __builtin_fmaf(cj[ii], o0[ii][u].x, gx);
+        gy = __builtin_fmaf(cj[ii], o0[ii][u].y, gy);
       }
+      const float2 nv = make_float2(xv[u].x + gx, xv[u].y + gy);
+      if (pr < DP) *reinterpret_cast<float2*>(xr + 2 * pr) = nv;
+      if (kn == kj) xn[u] = nv;
     }
 #pragma unroll
     for (int u = 0; u < NP; u++) xv[u] = xn[u];
+    ca = na;
+    cb = nb;
   }
 }
 
@@ -1120,24 +1114,16 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
     // or the LDS tile path (large windows / many negatives).
     const bool fast = nin <= 64 && os <= kOC;
     const int myk = s.ckey[min(lane, nin - 1)];
-    float cr[kOC];
     float2 o0r[kOC][NP], mdr[kOC][NP];
     if (fast) {
-      if (use_mfma) {
-        // GEMM1 on MFMA; lane j then takes its context's coefficients.
+      // GEMM1 leaves the coefficients in Ct[j][0..kOC).
+      if (use_mfma)
         mfma_gemm1<NP>(s, s.ring, RST, plan_meta(s, K, plan), D, os, nin, fin, alpha, expt, lane);
-        wave_sync();
-        const int CIS = cmf_stride(K);
-        const float4 c0 = *reinterpret_cast<const float4*>(s.Ct + (size_t)min(lane, nin - 1) * CIS);
-        const float4 c1 = *reinterpret_cast<const float4*>(s.Ct + (size_t)min(lane, nin - 1) * CIS + 4);
-        const float cc[kOC] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-#pragma unroll
-        for (int ii = 0; ii < kOC; ii++) cr[ii] = (lane < nin) ? cc[ii] : 0.f;
-      } else {
-        fast_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, os, nin, fin, alpha, expt, lane, myk, cr);
-      }
+      else
+        fast_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, os, nin, fin, alpha, expt, lane, myk);
+      wave_sync();
       load_o0<NP>(s, D, lane, o0r);
-      fast_gemm23<NP>(s.ring, RST, D, os, nin, lane, myk, cr, o0r, mdr, !has_dup);
+      fast_gemm23<NP>(s.ring, RST, s.Ct, D, nin, lane, myk, o0r, mdr, !has_dup);
     } else {
       step_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, K, os, nin, fin, alpha, expt, lane);
     }
@@ -1208,7 +1194,7 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
     // update is added into its ring row in context order, so repeated
     // contexts accumulate as in the reference's sentence buffer.
     if (fast) {
-      if (has_dup) fast_gemm3<NP>(s.ring, RST, D, os, nin, lane, myk, cr, o0r);
+      if (has_dup) fast_gemm3<NP>(s.ring, RST, s.Ct, D, nin, lane, myk, o0r);
     } else {
       float2 o0[kOC][NP];
       load_o0<NP>(s, D, lane, o0);
