@@ -135,6 +135,7 @@ struct Lds {
   int* negw;    // K*Lmax negative draws
   int* negs;    // K*Lmax: first draw of the same rank
   unsigned* nflag;  // bit per negative-buffer row: the row holds its snapshot
+  unsigned* nbloom; // kBloomWords: bit hash(rank) set for every rank of the negative batch
   int* cpos;    // 4W context positions
   int* csent;   // 4W context sentences
   int* ckey;    // 4W ring rows of the contexts (q*RR + entry)
@@ -155,6 +156,8 @@ __host__ __device__ inline int o_stride(int D) { return (D + 3) & ~3; }
 __host__ __device__ inline int ring_stride(int D) { return ((D + 3) & ~3) + 4; }
 __host__ __device__ inline int c_stride(int K) { return ko8(K); }
 __host__ __device__ inline int nflag_words(int K, int Lmax) { return (K * Lmax + 31) / 32; }
+constexpr int kBloomWords = 64;
+__host__ __device__ inline unsigned bloom_bit(int rank) { return ((unsigned)rank * 0x9E3779B1u) >> 21; }
 // MFMA path (at most kOC output rows): coefficient tile Cji[j][i],
 // 32*ceil(4W/32) context rows of kOC output columns.
 __host__ __device__ inline int cmf_stride(int K) { (void)K; return kOC; }
@@ -168,7 +171,7 @@ __host__ __device__ inline size_t lds_layout(int Lmax, int K, int W, int D, size
   const int KO = ko8(K), MC = 4 * W, RR = 2 * W + 1;
   size_t o = 0;
   off[0] = o; o = align16(o + 10 * (size_t)Lmax * 4);      // tok, slot, nxt, prv, pent (x2)
-  off[1] = o; o = align16(o + (2 * (size_t)K * Lmax + nflag_words(K, Lmax)) * 4);  // negw, negs, nflag
+  off[1] = o; o = align16(o + (2 * (size_t)K * Lmax + nflag_words(K, Lmax) + kBloomWords) * 4);  // negw, negs, nflag, nbloom
   off[2] = o; o = align16(o + 3 * (size_t)MC * 4);         // cpos, csent, ckey
   off[3] = o; o = align16(o + 6 * (size_t)KO * 4);         // two output plans
   off[4] = o; o = align16(o + 2 * (size_t)RR * ring_stride(D) * 4);
@@ -187,6 +190,7 @@ __device__ inline Lds carve(unsigned char* smem, int Lmax, int K, int W, int D) 
   int* b1 = reinterpret_cast<int*>(smem + off[1]);
   s.negw = b1; s.negs = b1 + K * Lmax;
   s.nflag = reinterpret_cast<unsigned*>(b1 + 2 * K * Lmax);
+  s.nbloom = s.nflag + nflag_words(K, Lmax);
   int* b2 = reinterpret_cast<int*>(smem + off[2]);
   s.cpos = b2; s.csent = b2 + MC; s.ckey = b2 + 2 * MC;
   int* b3 = reinterpret_cast<int*>(smem + off[3]);
@@ -216,6 +220,32 @@ __device__ inline void first_occurrence(const int* v, int* first, int n, int lan
     }
     if (p < n) first[p] = f;
   }
+}
+
+// first[p] as above through an LDS hash table tab of 2^cap_log2 words, each
+// (value << 11 | first position); the smallest position of a value wins by
+// atomic min.  Needs values < 2^21 and n <= 3/4 of the table (<= 2048).
+__device__ inline void first_occurrence_hash(const int* v, int* first, int n, int lane, unsigned* tab, int cap_log2) {
+  const unsigned cap = 1u << cap_log2, empty = 0xFFFFFFFFu;
+  for (unsigned x = lane; x < cap; x += 64) tab[x] = empty;
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  for (int c0 = 0; c0 < n; c0 += 64) {
+    const int p = c0 + lane;
+    if (p < n) {
+      const unsigned key = (unsigned)v[p];
+      const unsigned pk = (key << 11) | (unsigned)p;
+      unsigned h = (key * 0x9E3779B1u) >> (32 - cap_log2);
+      for (;;) {
+        const unsigned old = atomicCAS(&tab[h], empty, pk);
+        if (old == empty) break;
+        if ((old >> 11) == key) { atomicMin(&tab[h], pk); break; }
+        h = (h + 1) & (cap - 1);
+      }
+      first[p] = (int)h;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  for (int p = lane; p < n; p += 64) first[p] = (int)(tab[first[p]] & 0x7FFu);
 }
 
 // For each position p < n: first, previous and next position holding the
@@ -392,15 +422,18 @@ __device__ inline void step_gemm3_batch(const Lds& s, int D, int K, int os, int 
   }
 }
 
+// Rows i >= os are zero: the staged-output area doubles as scratch between
+// steps, and GEMM3 multiplies every row (by a zero coefficient past os).
 template <int NP>
-__device__ inline void load_o0(const Lds& s, int D, int lane, float2 (&o0)[kOC][NP]) {
+__device__ inline void load_o0(const Lds& s, int D, int os, int lane, float2 (&o0)[kOC][NP]) {
   const int OST = o_stride(D), DP = D >> 1;
 #pragma unroll
   for (int ii = 0; ii < kOC; ii++)
 #pragma unroll
     for (int u = 0; u < NP; u++) {
       const int pr = lane + 64 * u;
-      o0[ii][u] = (pr < DP) ? *reinterpret_cast<const float2*>(s.O + ii * OST + 2 * pr) : make_float2(0.f, 0.f);
+      o0[ii][u] = (ii < os && pr < DP) ? *reinterpret_cast<const float2*>(s.O + ii * OST + 2 * pr)
+                                       : make_float2(0.f, 0.f);
     }
 }
 
@@ -665,6 +698,8 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
   const Lds s = carve(smem + exp_lds_bytes() + (size_t)wave * lds_layout(Lmax, K, W, D, off_unused), Lmax, K, W, D);
   constexpr int B = (32 / NP) > 2 ? (32 / NP) : 2;  // rows with loads in flight per batch
   const bool can_plan = (K + 2 <= kOC);              // next-step outputs fit one register chunk
+  int tab_log2 = 0;  // largest power-of-two table in the staged-output + coefficient areas
+  while ((2ull << tab_log2) <= (unsigned long long)KO * OST + cmat_floats(K, W)) tab_log2++;
   // The BF16 form runs the three contractions on MFMA where the shapes allow.
   const bool use_mfma = std::is_same<R, uint16_t>::value && a.mfma && mfma_ok(D, K, W);
   // Output rows beyond the current step's count are read (and their results
@@ -958,7 +993,18 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
         rng = f;
       }
       wave_sync();
-      first_occurrence(s.negw, s.negs, NN, lane);
+      // The staged-output and coefficient areas (contiguous, free here) hold
+      // the hash table.
+      if (a.V <= (1 << 21) && 4 * NN <= 3 * (1 << tab_log2))
+        first_occurrence_hash(s.negw, s.negs, NN, lane, reinterpret_cast<unsigned*>(s.O), tab_log2);
+      else
+        first_occurrence(s.negw, s.negs, NN, lane);
+      for (int w = lane; w < kBloomWords; w += 64) s.nbloom[w] = 0u;
+      wave_sync();
+      for (int n = lane; n < NN; n += 64) {
+        const unsigned h = bloom_bit(s.negw[n]);
+        atomicOr(&s.nbloom[h >> 5], 1u << (h & 31));
+      }
       wave_sync();
       // The negative buffer holds a snapshot of each row taken at the refresh
       // (dsgl.hpp:1081-1084).  Rows are snapshotted lazily: at first use as a
@@ -1018,16 +1064,18 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
     }
     wave_sync();
     // Repeated contexts share a ring row; the reduction path below keeps the
-    // reference's order for them.
+    // reference's order for them.  Each context writes its index to a tag
+    // for its ring row (coefficient area, free until GEMM1) and reads it
+    // back: another index there means another context shares the row.
+    int* const tag = reinterpret_cast<int*>(s.Ct);
+    for (int j = lane; j < nin; j += 64) tag[s.ckey[j]] = j;
+    wave_sync();
     bool has_dup = false;
     for (int j0 = 0; j0 < nin; j0 += 64) {
       const int j = j0 + lane;
-      const int key = (j < nin) ? s.ckey[j] : -1;
-      bool dup = false;
-#pragma unroll 8
-      for (int j2 = 0; j2 < nin; j2++) dup = dup || (j2 < j && s.ckey[j2] == key);
-      has_dup = has_dup || (__ballot(j < nin && dup) != 0);
+      has_dup = has_dup || (__ballot(j < nin && tag[s.ckey[min(j, nin - 1)]] != j) != 0);
     }
+    wave_sync();
     // Output rows: prefetched by the previous step, or loaded now.
     if (planned) {
 #pragma unroll
@@ -1072,15 +1120,20 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
         }
       }
     }
-    if (!planned)
-      for (int i = 1; i < os - 1; i++) nmark(plan_slot(s, K, plan)[i]);
+    if (!planned && lane > 0 && lane < os - 1) {
+      const int sl = plan_slot(s, K, plan)[lane];
+      atomicOr(&s.nflag[sl >> 5], 1u << (sl & 31));
+    }
     // Targets that are pending negatives take their snapshot now, before this
     // step's update of the row.
+    const unsigned hb1 = bloom_bit(t1), hb2 = bloom_bit(t2);
+    const bool maybe1 = (s.nbloom[hb1 >> 5] >> (hb1 & 31)) & 1u;
+    const bool maybe2 = (s.nbloom[hb2 >> 5] >> (hb2 & 31)) & 1u;
 #pragma unroll
     for (int tq = 0; tq < 2; tq++) {
       const int r = tq ? t2 : t1;
       int sl = -1;
-      for (int c0 = 0; c0 < nbn * K && sl < 0; c0 += 64) {
+      for (int c0 = 0; c0 < nbn * K && sl < 0 && (tq ? maybe2 : maybe1); c0 += 64) {
         const int n = c0 + lane;
         const u64 hit = __ballot(n < nbn * K && s.negw[n] == r);
         if (hit) sl = s.negs[c0 + __ffsll((long long)hit) - 1];
@@ -1122,7 +1175,7 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
       else
         fast_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, os, nin, fin, alpha, expt, lane, myk);
       wave_sync();
-      load_o0<NP>(s, D, lane, o0r);
+      load_o0<NP>(s, D, os, lane, o0r);
       fast_gemm23<NP>(s.ring, RST, s.Ct, D, nin, lane, myk, o0r, mdr, !has_dup);
     } else {
       step_gemm1(s, s.ring, RST, plan_meta(s, K, plan), D, K, os, nin, fin, alpha, expt, lane);
@@ -1169,14 +1222,17 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
       const int t1n = s.tok[p0 + 1], t2n = s.tok[Lmax + p1 + 1];
       const int osn = make_outputs(np, t1n, t2n, (neg_count + 1) % nbn);
       wave_sync();
-      pre_nb = 0;
+      // Lane i < osn resolves row i's source, then every row is read.
+      const bool neg_row = lane > 0 && lane < osn - 1;
+      const int lsl = neg_row ? plan_slot(s, K, np)[lane] : 0;
+      const int lrk = (lane == 0) ? t1n : (lane == osn - 1) ? t2n : (neg_row ? plan_idx(s, K, np)[lane] : 0);
+      pre_nb = (unsigned)__ballot(neg_row && nsnap(lsl));
 #pragma unroll
       for (int i = 0; i < kOC; i++) {
         if (i < osn) {
-          const int sl = plan_slot(s, K, np)[i];
-          const bool from_nb = i > 0 && i < osn - 1 && nsnap(sl);
-          const int rk = (i == 0) ? t1n : (i == osn - 1) ? t2n : plan_idx(s, K, np)[i];
-          pre_nb |= from_nb ? (1u << i) : 0u;
+          const int sl = __builtin_amdgcn_readlane(lsl, i);
+          const int rk = __builtin_amdgcn_readlane(lrk, i);
+          const bool from_nb = (pre_nb >> i) & 1u;
 #pragma unroll
           for (int u = 0; u < NP; u++) {
             const int pr = lane + 64 * u;
@@ -1187,7 +1243,7 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
         }
       }
       pre_os = osn;
-      for (int i = 1; i < osn - 1; i++) nmark(plan_slot(s, K, np)[i]);
+      if (neg_row) atomicOr(&s.nflag[lsl >> 5], 1u << (lsl & 31));
     }
 
     // GEMM3 and the context scatter (dsgl.hpp:1314-1333): each context's
@@ -1197,7 +1253,7 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
       if (has_dup) fast_gemm3<NP>(s.ring, RST, s.Ct, D, nin, lane, myk, o0r);
     } else {
       float2 o0[kOC][NP];
-      load_o0<NP>(s, D, lane, o0);
+      load_o0<NP>(s, D, os, lane, o0);
       for (int j0 = 0; j0 < nin; j0 += 8) {
         float2 g[8][NP];
         step_gemm3_batch<NP>(s, D, K, os, nin, j0, lane, o0, g);
@@ -1436,7 +1492,7 @@ __global__ __launch_bounds__(64) void k_step(int D, int os, int nin, int fin, co
   step_gemm2<NP>(s, s.ring, RST, D, K, os, nin, lane,
                  [&](int i, int pr, float2 v) { st2(md + (size_t)i * D + 2 * pr, v); });
   float2 o0[kOC][NP];
-  load_o0<NP>(s, D, lane, o0);
+  load_o0<NP>(s, D, os, lane, o0);
   for (int j0 = 0; j0 < nin; j0 += 8) {
     float2 g[8][NP];
     step_gemm3_batch<NP>(s, D, K, os, nin, j0, lane, o0, g);
