@@ -9,11 +9,17 @@ with about 1e8 walk tokens of log-normal(3.7, 0.5) lengths clipped to
 [5, 120]; dim 128, window 10, negative 5, sample 1e-3, alpha 0.01.
 
 The printed JSON line reports trained words/s (post-subsampling target
-positions) of the BF16-row form as ``value``, the FP32 form beside it, the
-link-prediction AUC of each form after one pass, the roofline of the trainer
-kernel from HIP-event kernel time and algorithmic bytes, and the CPU baseline
-(the oracle's OpenMP Hogwild trainer on the host cores, on a bounded prefix
-sample of the same corpus).
+positions) of the BF16-MFMA form as ``value``, the plain FP32 and BF16-FMA
+forms beside it, the link-prediction AUC of each form after one pass, the
+roofline of the trainer kernel from HIP-event kernel time and algorithmic
+bytes (HBM traffic per launch from the PMC summary committed under profiles/,
+when one exists for this workload and form), and the CPU baseline (the
+oracle's OpenMP Hogwild trainer on the host cores, on a bounded prefix sample
+of the same corpus).
+
+--copy-calibration MIB runs one device-to-device copy of that size before the
+workload: the known-size launch that scales the FETCH_SIZE / WRITE_SIZE
+counters of a PMC run.
 """
 from __future__ import annotations
 
@@ -29,6 +35,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gdistger-re_amd"))
 
 HBM_PEAK_GBS = 8000.0
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+METRIC = "trained words/sec, BF16-MFMA vs FP32 plain, at \u22640.005 link-pred AUC gap"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_train_rmat21_traffic.json")
 
 
 def log(msg: str) -> None:
@@ -55,8 +65,10 @@ def main() -> int:
     ap.add_argument("--forms", default="fp32,bf16,bf16-mfma")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-auc", action="store_true")
-    ap.add_argument("--cpu-tokens", type=float, default=16e6,
+    ap.add_argument("--cpu-tokens", type=float, default=40e6,
                     help="prefix of the corpus the CPU baseline trains on")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="MiB of device-to-device copy to run first (PMC calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -68,6 +80,14 @@ def main() -> int:
     from dger_sgns import synth
 
     D, W, K, sample, alpha = 128, 10, 5, 1e-3, 0.01
+    if args.copy_calibration > 0:
+        n = int(args.copy_calibration * (1 << 20))
+        src = G.DeviceArray(np.zeros(n, np.uint8))
+        dst = G.DeviceArray.zeros(n, np.uint8)
+        dst.copy_from(src)
+        log(f"copy calibration: {n} bytes device to device")
+        src.free()
+        dst.free()
     t0 = time.time()
     corpus, test_pairs, test_labels = synth.rmat_corpus(
         scale=args.scale, avg_degree=16, abc=(0.57, 0.19, 0.19), seed=3,
@@ -87,23 +107,31 @@ def main() -> int:
             st, _, _ = tr.train()
             log(f"[{form}] warmup {i}: {st.trained_words} words, {st.kernel_ms:.1f} ms kernel")
         words = 0
+        raw = 0
         kms = []
         wall0 = time.perf_counter()
         for i in range(args.steps):
             st, _, _ = tr.train()
             words += st.trained_words
+            raw += st.raw_words
             kms.append(st.kernel_ms)
             log(f"[{form}] step {i}: {st.trained_words} words, {st.kernel_ms:.1f} ms kernel, "
                 f"{st.words_per_sec / 1e6:.1f} M words/s, workers {st.workers}")
         wall = time.perf_counter() - wall0
+        kavg = float(np.mean(kms)) if kms else 0.0
+        flops = 6.0 * D * float(st.out_ctx)  # three contractions, 2 flops per multiply-add
         res = {
             "words_per_sec": words / wall,
+            "raw_tokens_per_sec": raw / wall,
             "ms_per_step": 1e3 * wall / max(1, args.steps),
-            "kernel_ms_avg": float(np.mean(kms)) if kms else 0.0,
+            "kernel_ms_avg": kavg,
             "trained_words_per_step": int(st.trained_words),
             "raw_words_per_step": int(st.raw_words),
             "workers": int(st.workers),
             "max_sentence": int(st.max_sentence),
+            "S": 2,
+            "C_bar": float(st.contexts) / max(1, st.trained_words),
+            "contraction_tflops": flops / (kavg * 1e-3) / 1e12 if kavg else 0.0,
         }
         if not args.no_auc:
             tr.reset_model()
@@ -116,13 +144,15 @@ def main() -> int:
         results[form] = res
 
     bf16_forms = [f for f in ("bf16-mfma", "bf16") if f in results]
-    head = max(bf16_forms, key=lambda f: results[f]["words_per_sec"]) if bf16_forms else forms[0]
+    head = "bf16-mfma" if "bf16-mfma" in results else (bf16_forms[0] if bf16_forms else forms[0])
     h = results[head]
     row_bytes = 2 if head.startswith("bf16") else 4
     bpw = algorithmic_bytes_per_word(D, K, row_bytes)
     achieved = h["trained_words_per_step"] * bpw / (h["kernel_ms_avg"] * 1e-3) / 1e9
+    workload = f"rmat-s{args.scale}-deg16-lognormal3.7-{corpus.n_tokens}tok"
+    peak_tf = BF16_MFMA_PEAK_TFLOPS if head == "bf16-mfma" else FP32_PEAK_TFLOPS
     out = {
-        "metric": "trained words/sec, BF16 (MFMA or FMA contractions) vs FP32 plain, at <=0.005 link-pred AUC gap",
+        "metric": METRIC,
         "value": h["words_per_sec"],
         "unit": "words/s",
         "n_gpus": 1,
@@ -135,7 +165,7 @@ def main() -> int:
         "dtype": "bf16" if head.startswith("bf16") else "fp32",
         "data": "synthetic (R-MAT graph, uniform random walks; seeded)",
         "config": {
-            "workload": f"rmat-s{args.scale}-deg16-lognormal3.7-{corpus.n_tokens}tok",
+            "workload": workload,
             "dim": D, "window": W, "negative": K, "sample": sample, "alpha": alpha, "iter": 1,
             "tokens": corpus.n_tokens, "walks": corpus.n_walks, "vocab": int(corpus.vertex_cn.size),
             "workers": h["workers"], "form": head,
@@ -148,8 +178,15 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(workload, head, h["trained_words_per_step"]),
             "bytes_per_word": bpw,
+            "S": 2,
+            "C_bar": h["C_bar"],
+        },
+        "contractions": {
+            "tflops": h["contraction_tflops"],
+            "peak": peak_tf,
+            "frac": h["contraction_tflops"] / peak_tf,
         },
     }
     if "fp32" in results and head != "fp32":
@@ -161,6 +198,22 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(corpus, D, W, K, sample, alpha, int(args.cpu_tokens))
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measured_traffic(workload, form, trained_words):
+    """HBM bytes per launch from the committed PMC summary (FETCH_SIZE +
+    WRITE_SIZE of the trainer, scaled by the copy-kernel calibration), as
+    bytes per trained word times this launch's trained words; None if the
+    summary does not cover this workload and form."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get("forms", {}).get(form)
+    if t.get("workload") != workload or not e:
+        return None
+    return e["hbm_bytes_per_word"] * trained_words
 
 
 def cpu_baseline(corpus, D, W, K, sample, alpha, n_tokens):

@@ -478,6 +478,8 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
     stats->trained_words = 2 * cnt[dger::kSteps];
     stats->raw_words = cnt[dger::kRawWords];
     stats->sentences = cnt[dger::kSentences];
+    stats->contexts = cnt[dger::kContexts];
+    stats->out_ctx = cnt[dger::kOutCtx];
     stats->final_alpha = alpha0;
     stats->workers = workers;
     stats->vocab_size = (int32_t)c->V;
@@ -688,6 +690,11 @@ int dger_dev_download(void* dst, const void* src, uint64_t bytes) {
 }
 int dger_dev_zero(void* ptr, uint64_t bytes) {
   DGER_HIP(hipMemset(ptr, 0, bytes));
+  return DGER_OK;
+}
+int dger_dev_copy(void* dst, const void* src, uint64_t bytes) {
+  if ((!dst || !src) && bytes) return fail(DGER_E_ARG, "null pointer");
+  DGER_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
   return DGER_OK;
 }
 int dger_device_count(int32_t* count) {

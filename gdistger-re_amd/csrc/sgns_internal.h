@@ -18,7 +18,9 @@ enum Counter : int {
   kSteps = 1,
   kRawWords = 2,
   kSentences = 3,
-  kNumCounters = 4,
+  kContexts = 4,   // sum over steps of the context count
+  kOutCtx = 5,     // sum over steps of (output rows x contexts): MACs per contraction / D
+  kNumCounters = 6,
 };
 
 struct TrainArgs {

@@ -735,7 +735,7 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
   u64 wc = 0, last_wc = 0, cur = start;
   int len[2] = {0, 0}, pos[2] = {0, 0};
   int done_sent = 0, refresh = 1, neg_count = 0, nbn = 1;
-  u64 steps = 0, raw = 0, sentences = 0;
+  u64 steps = 0, raw = 0, sentences = 0, n_ctx = 0, n_outctx = 0;
   const int stride = record_stride(W, K);
   int32_t* const rec = a.rec ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
   int plan = 0;           // which output plan the step uses
@@ -1300,6 +1300,8 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
       }
     }
     steps++;
+    n_ctx += (u64)nin;
+    n_outctx += (u64)(os * nin);
     wave_sync();
 
     // Transition: advance both sentences; a finished sentence writes its
@@ -1409,6 +1411,8 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
     atomicAdd(&a.counters[kSteps], steps);
     atomicAdd(&a.counters[kRawWords], raw);
     atomicAdd(&a.counters[kSentences], sentences);
+    atomicAdd(&a.counters[kContexts], n_ctx);
+    atomicAdd(&a.counters[kOutCtx], n_outctx);
   }
 }
 

@@ -63,7 +63,7 @@ class Stats(C.Structure):
         ("steps", C.c_uint64), ("trained_words", C.c_uint64), ("raw_words", C.c_uint64),
         ("sentences", C.c_uint64), ("final_alpha", C.c_float), ("workers", C.c_int32),
         ("vocab_size", C.c_int32), ("max_sentence", C.c_int32), ("kernel_ms", C.c_double),
-        ("words_per_sec", C.c_double),
+        ("words_per_sec", C.c_double), ("contexts", C.c_uint64), ("out_ctx", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -100,6 +100,7 @@ _SIGS = {
     "dger_dev_upload": (C.c_int, [_vp, _vp, C.c_uint64]),
     "dger_dev_download": (C.c_int, [_vp, _vp, C.c_uint64]),
     "dger_dev_zero": (C.c_int, [_vp, C.c_uint64]),
+    "dger_dev_copy": (C.c_int, [_vp, _vp, C.c_uint64]),
     "dger_device_count": (C.c_int, [C.POINTER(_i32)]),
 }
 EXPORTS = tuple(_SIGS)
@@ -309,6 +310,12 @@ class DeviceArray:
 
     def zero(self):
         _check(lib().dger_dev_zero(self.ptr, self.nbytes))
+
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy of other's bytes into this allocation."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _check(lib().dger_dev_copy(self.ptr, other.ptr, self.nbytes))
 
     def get(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)

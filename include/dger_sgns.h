@@ -86,6 +86,9 @@ typedef struct dger_sgns_stats {
   int32_t max_sentence;    /* longest sentence slot the kernels were sized for */
   double kernel_ms;        /* HIP-event time of the trainer kernel launches */
   double words_per_sec;    /* trained_words / kernel time */
+  uint64_t contexts;       /* sum over steps of the contexts of both windows */
+  uint64_t out_ctx;        /* sum over steps of (output rows x contexts); each of the three
+                              contractions is D * out_ctx multiply-adds */
 } dger_sgns_stats;
 
 /* Fills *out with the reference defaults (dsgl.hpp:67-76) and GPU defaults. */
@@ -188,6 +191,8 @@ int dger_dev_free(void *ptr);
 int dger_dev_upload(void *dst, const void *src, uint64_t bytes);
 int dger_dev_download(void *dst, const void *src, uint64_t bytes);
 int dger_dev_zero(void *ptr, uint64_t bytes);
+/* Device-to-device copy (also the known-size reference for traffic counters). */
+int dger_dev_copy(void *dst, const void *src, uint64_t bytes);
 int dger_device_count(int32_t *count);
 
 #ifdef __cplusplus
