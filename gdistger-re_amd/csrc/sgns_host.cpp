@@ -141,8 +141,20 @@ struct dger_ctx {
   uint64_t n_tokens = 0;
   int Lmax = 0;
   bool have_corpus = false;
+  // Captured trainer launch, valid while its arguments are unchanged.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  dger::TrainArgs graph_args{};
+  int tuned_workers = 0;  // autotuner's choice (0 = not tuned)
 
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
   ~dger_ctx() {
+    drop_graph();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -178,6 +190,8 @@ int dger_sgns_default_opts(dger_sgns_opts* o) {
   o->precision = DGER_PRECISION_FP32;
   o->table_size = 100000000LL;
   o->device = 0;
+  o->graph = 1;
+  o->autotune = 0;
   return DGER_OK;
 }
 
@@ -234,6 +248,8 @@ int dger_sgns_parse_args(int argc, char** argv, dger_sgns_opts* o) {
   DGER_STR("-save-emb", save_emb)
   DGER_INT("-workers", workers)
   DGER_INT("-device", device)
+  DGER_INT("-graph", graph)
+  DGER_INT("-autotune", autotune)
   if ((v = val("-precision"))) {
     if (!*v) return missing("-precision");
     if (!std::strcmp(v, "bf16") || !std::strcmp(v, "1")) o->precision = DGER_PRECISION_BF16;
@@ -389,6 +405,119 @@ static int auto_workers(dger_ctx* c, size_t block_lds, int wpb) {
   return std::max(1, w);
 }
 
+static int ensure_worker_buffers(dger_ctx* c, int workers) {
+  const size_t per = dger::worker_scratch_floats(c->Lmax, c->o.negative, c->D);
+  if (c->scratch_d.ensure((size_t)workers * per * 4) || c->wsteps_d.ensure((size_t)workers * 8) ||
+      c->walpha_d.ensure((size_t)workers * 4))
+    return fail(DGER_E_ALLOC, "scratch allocation failed");
+  return DGER_OK;
+}
+
+// Kernel arguments of one training pass over the first n_tokens of the corpus
+// (value-initialised so that equal passes compare equal byte for byte).
+static dger::TrainArgs make_args(dger_ctx* c, int workers, int wpb, int32_t* rec, int64_t rec_cap,
+                                 uint64_t n_tokens) {
+  dger::TrainArgs a{};
+  a.corpus = c->corpus_d.as<int32_t>();
+  a.n_tokens = n_tokens;
+  a.id_rank = c->id_rank_d.as<int32_t>();
+  a.keep_thr = c->keep_d.as<float>();
+  a.table = c->table_d.as<int32_t>();
+  a.table_size = (uint64_t)c->o.table_size;
+  a.exp_table = c->exp_d.as<float>();
+  a.wih = c->wih_d.p;
+  a.woh = c->woh_d.p;
+  a.fscratch = c->scratch_d.as<float>();
+  a.counters = c->counters_d.as<unsigned long long>();
+  a.rec = rec;
+  a.rec_cap = rec_cap;
+  a.worker_steps = c->wsteps_d.as<unsigned long long>();
+  a.worker_alpha = c->walpha_d.as<float>();
+  a.V = (int32_t)c->V;
+  a.D = c->D;
+  a.W = c->o.window;
+  a.K = c->o.negative;
+  a.iter = c->o.iter;
+  a.workers = workers;
+  a.Lmax = c->Lmax;
+  a.wpb = wpb;
+  a.sample_on = c->o.sample > 0 ? 1 : 0;
+  a.mfma = c->o.precision == DGER_PRECISION_BF16_MFMA ? 1 : 0;
+  a.alpha0 = c->o.alpha;
+  a.alpha_den = (float)((uint64_t)c->o.iter * c->train_words + 1);
+  return a;
+}
+
+// One pass: counter reset (a fill kernel, so that the pair can be captured)
+// and the trainer kernel, timed with events on the context's stream.  With
+// use_graph the pair is captured once and the graph replayed while the
+// arguments stay the same.
+static int run_pass(dger_ctx* c, const dger::TrainArgs& a, bool use_graph, float* ms) {
+  DGER_HIP(dger::prepare_train(a, c->o.precision));
+  if (use_graph && (!c->graph_exec || std::memcmp(&c->graph_args, &a, sizeof(a)) != 0)) {
+    c->drop_graph();
+    DGER_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = dger::launch_zero_u64(a.counters, dger::kNumCounters, c->stream);
+    if (e == hipSuccess) e = dger::launch_train(a, c->o.precision, c->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      c->graph_exec = nullptr;
+      return fail(DGER_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    c->graph = g;
+    c->graph_args = a;
+  }
+  DGER_HIP(hipEventRecord(c->ev0, c->stream));
+  if (use_graph) {
+    DGER_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+  } else {
+    DGER_HIP(dger::launch_zero_u64(a.counters, dger::kNumCounters, c->stream));
+    DGER_HIP(dger::launch_train(a, c->o.precision, c->stream));
+  }
+  DGER_HIP(hipEventRecord(c->ev1, c->stream));
+  DGER_HIP(hipStreamSynchronize(c->stream));
+  DGER_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return DGER_OK;
+}
+
+// Worker-count autotuner: times one pass over a corpus prefix for one to four
+// workers per CU (one workgroup per CU) and returns the fastest in trained
+// words per second.  The model is saved before and restored after.
+static int autotune_workers(dger_ctx* c, int cus, int wpb_max, int* out) {
+  const uint64_t n = (uint64_t)c->V * (uint64_t)c->D * (c->o.precision == DGER_PRECISION_FP32 ? 4 : 2);
+  DevBuf save_i, save_o;
+  if (save_i.ensure(n) || save_o.ensure(n)) return fail(DGER_E_ALLOC, "autotune: model copy allocation failed");
+  DGER_HIP(hipMemcpyAsync(save_i.p, c->wih_d.p, n, hipMemcpyDeviceToDevice, c->stream));
+  DGER_HIP(hipMemcpyAsync(save_o.p, c->woh_d.p, n, hipMemcpyDeviceToDevice, c->stream));
+  const uint64_t sample = std::min<uint64_t>(c->n_tokens, 16ull << 20);
+  double best = -1.0;
+  int best_w = 0;
+  for (int per_cu = 1; per_cu <= std::min(4, wpb_max); per_cu++) {
+    const int w = std::max<int>(1, std::min<uint64_t>((uint64_t)cus * per_cu, std::max<uint64_t>(1, sample / 4096)));
+    if (const int rc = ensure_worker_buffers(c, w); rc != DGER_OK) return rc;
+    const dger::TrainArgs a = make_args(c, w, std::min(per_cu, w), nullptr, 0, sample);
+    float ms = 0.f;
+    if (const int rc = run_pass(c, a, false, &ms); rc != DGER_OK) return rc;
+    unsigned long long steps = 0;
+    DGER_HIP(hipMemcpy(&steps, c->counters_d.as<unsigned long long>() + dger::kSteps, 8, hipMemcpyDeviceToHost));
+    const double rate = ms > 0 ? 2.0 * (double)steps / ms : 0.0;
+    if (c->o.debug > 2) fprintf(stderr, "dger: autotune %d workers: %.1f M words/s\n", w, rate * 1e-3);
+    if (rate > best) {
+      best = rate;
+      best_w = w;
+    }
+  }
+  DGER_HIP(hipMemcpyAsync(c->wih_d.p, save_i.p, n, hipMemcpyDeviceToDevice, c->stream));
+  DGER_HIP(hipMemcpyAsync(c->woh_d.p, save_o.p, n, hipMemcpyDeviceToDevice, c->stream));
+  DGER_HIP(hipStreamSynchronize(c->stream));
+  *out = best_w;
+  return DGER_OK;
+}
+
 int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, uint64_t* worker_steps,
                    dger_sgns_stats* stats) {
   if (!c) return fail(DGER_E_ARG, "null ctx");
@@ -407,7 +536,14 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
                     std::to_string(dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D,
                                                                c->o.precision, 1)) +
                     " bytes of LDS, more than the device allows");
+  const int wpb_max = wpb;
   if (workers <= 0) workers = c->o.workers;
+  if (workers <= 0 && c->tuned_workers > 0) workers = c->tuned_workers;
+  if (workers <= 0 && c->o.autotune && !rec) {
+    const int rc = autotune_workers(c, prop.multiProcessorCount, wpb_max, &workers);
+    if (rc != DGER_OK) return rc;
+    c->tuned_workers = workers;
+  }
   if (workers <= 0)
     workers = auto_workers(
         c, dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision, wpb), wpb);
@@ -417,50 +553,15 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
             workers, wpb,
             dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision, wpb),
             (size_t)prop.sharedMemPerBlock, (size_t)prop.sharedMemPerBlockOptin);
-  const size_t per = dger::worker_scratch_floats(c->Lmax, c->o.negative, c->D);
   const int stride = dger::record_stride(c->o.window, c->o.negative);
-  if (c->scratch_d.ensure((size_t)workers * per * 4) || c->wsteps_d.ensure((size_t)workers * 8) ||
-      c->walpha_d.ensure((size_t)workers * 4))
-    return fail(DGER_E_ALLOC, "scratch allocation failed");
+  if (const int rc = ensure_worker_buffers(c, workers); rc != DGER_OK) return rc;
   if (rec && c->rec_d.ensure((size_t)workers * (size_t)rec_cap * stride * 4))
     return fail(DGER_E_ALLOC, "record allocation failed");
-  DGER_HIP(hipMemsetAsync(c->counters_d.p, 0, dger::kNumCounters * 8, c->stream));
 
-  dger::TrainArgs a;
-  a.corpus = c->corpus_d.as<int32_t>();
-  a.n_tokens = c->n_tokens;
-  a.id_rank = c->id_rank_d.as<int32_t>();
-  a.keep_thr = c->keep_d.as<float>();
-  a.table = c->table_d.as<int32_t>();
-  a.table_size = (uint64_t)c->o.table_size;
-  a.exp_table = c->exp_d.as<float>();
-  a.wih = c->wih_d.p;
-  a.woh = c->woh_d.p;
-  a.fscratch = c->scratch_d.as<float>();
-  a.counters = c->counters_d.as<unsigned long long>();
-  a.rec = rec ? c->rec_d.as<int32_t>() : nullptr;
-  a.rec_cap = rec ? rec_cap : 0;
-  a.worker_steps = c->wsteps_d.as<unsigned long long>();
-  a.worker_alpha = c->walpha_d.as<float>();
-  a.V = (int32_t)c->V;
-  a.D = c->D;
-  a.W = c->o.window;
-  a.K = c->o.negative;
-  a.iter = c->o.iter;
-  a.workers = workers;
-  a.Lmax = c->Lmax;
-  a.wpb = wpb;
-  a.sample_on = c->o.sample > 0 ? 1 : 0;
-  a.mfma = c->o.precision == DGER_PRECISION_BF16_MFMA ? 1 : 0;
-  a.alpha0 = c->o.alpha;
-  a.alpha_den = (float)((uint64_t)c->o.iter * c->train_words + 1);
-
-  DGER_HIP(hipEventRecord(c->ev0, c->stream));
-  DGER_HIP(dger::launch_train(a, c->o.precision, c->stream));
-  DGER_HIP(hipEventRecord(c->ev1, c->stream));
-  DGER_HIP(hipStreamSynchronize(c->stream));
+  const dger::TrainArgs a = make_args(c, workers, wpb, rec ? c->rec_d.as<int32_t>() : nullptr, rec ? rec_cap : 0,
+                                      c->n_tokens);
   float ms = 0.f;
-  DGER_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (const int rc = run_pass(c, a, c->o.graph != 0 && !rec, &ms); rc != DGER_OK) return rc;
 
   unsigned long long cnt[dger::kNumCounters];
   DGER_HIP(hipMemcpy(cnt, c->counters_d.p, sizeof(cnt), hipMemcpyDeviceToHost));

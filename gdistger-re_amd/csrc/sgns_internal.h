@@ -69,7 +69,10 @@ DGER_HD inline int record_stride(int W, int K) { return 8 + 4 * W + 2 * (K + 2);
 hipError_t launch_init_net(float* wih, uint64_t n, int D, hipStream_t s);
 hipError_t launch_table_fill(int32_t* table, uint64_t table_size, const int64_t* starts, int n_starts,
                              hipStream_t s);
+hipError_t prepare_train(const TrainArgs& a, int precision);
 hipError_t launch_train(const TrainArgs& a, int precision, hipStream_t s);
+// Zeroes n <= 64 counters with a kernel (capturable in a graph).
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s);
 hipError_t launch_bf16_to_f32(const uint16_t* src, float* dst, uint64_t n, hipStream_t s);
 hipError_t launch_step(int D, int os, int nin, int fin, const int32_t* meta, float alpha,

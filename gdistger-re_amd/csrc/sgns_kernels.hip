@@ -1578,6 +1578,27 @@ __global__ __launch_bounds__(64) void k_loss_bwd(const T* syn0, const T* syn1, i
   }
 }
 
+// Lets the trainer instance for a's shape take its workgroup's LDS (above the
+// 64 KB default); done before a launch, outside any stream capture.
+template <typename R>
+hipError_t prepare_train_t(const TrainArgs& a, int precision) {
+  if (a.wpb < 1 || a.wpb > kMaxWpb) return hipErrorInvalidValue;
+  const int lds = (int)(exp_lds_bytes() + (size_t)a.wpb * train_lds_bytes(a.Lmax, a.K, a.W, a.D, precision));
+  switch ((a.D / 2 + 63) / 64) {
+#define DGER_CASE(X) \
+  case X:            \
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train<R, X>), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    DGER_CASE(1) DGER_CASE(2) DGER_CASE(3) DGER_CASE(4)
+    DGER_CASE(5) DGER_CASE(6) DGER_CASE(7) DGER_CASE(8)
+#undef DGER_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_zero_u64(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 template <typename R>
 hipError_t launch_train_t(const TrainArgs& a, hipStream_t st, int precision) {
   if (a.wpb < 1 || a.wpb > kMaxWpb) return hipErrorInvalidValue;
@@ -1585,14 +1606,10 @@ hipError_t launch_train_t(const TrainArgs& a, hipStream_t st, int precision) {
   const int np = (a.D / 2 + 63) / 64;
   dim3 grid((a.workers + a.wpb - 1) / a.wpb), block(64 * a.wpb);
   switch (np) {
-#define DGER_CASE(X)                                                                              \
-  case X: {                                                                                       \
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train<R, X>),        \
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
-    if (e != hipSuccess) return e;                                                                \
-    hipLaunchKernelGGL((k_train<R, X>), grid, block, lds, st, a);                                 \
-    break;                                                                                        \
-  }
+#define DGER_CASE(X)                                                      \
+  case X:                                                                 \
+    hipLaunchKernelGGL((k_train<R, X>), grid, block, lds, st, a);         \
+    break;
     DGER_CASE(1) DGER_CASE(2) DGER_CASE(3) DGER_CASE(4)
     DGER_CASE(5) DGER_CASE(6) DGER_CASE(7) DGER_CASE(8)
 #undef DGER_CASE
@@ -1639,6 +1656,18 @@ hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipSt
 
 hipError_t launch_bf16_to_f32(const uint16_t* src, float* dst, uint64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_bf16_to_f32, dim3(4096), dim3(256), 0, s, src, dst, (u64)n);
+  return hipGetLastError();
+}
+
+hipError_t prepare_train(const TrainArgs& a, int precision) {
+  if (precision == DGER_PRECISION_BF16 || precision == DGER_PRECISION_BF16_MFMA)
+    return prepare_train_t<uint16_t>(a, precision);
+  return prepare_train_t<float>(a, precision);
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
+  if (n < 0 || n > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p, n);
   return hipGetLastError();
 }
 

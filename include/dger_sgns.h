@@ -73,6 +73,10 @@ typedef struct dger_sgns_opts {
   int32_t precision;       /* -precision   fp32 (0), bf16 (1), bf16-mfma (2) */
   int64_t table_size;      /* -table-size  noise-table slots (1e8, dsgl.hpp:82) */
   int32_t device;          /* -device      HIP device ordinal (0) */
+  int32_t graph;           /* -graph       1: the trainer launch (counter reset + kernel) is captured
+                              once in a HIP graph and replayed by later passes (1) */
+  int32_t autotune;        /* -autotune    1: with -workers 0, the first pass times candidate
+                              worker counts on a corpus prefix and keeps the fastest (0) */
 } dger_sgns_opts;
 
 typedef struct dger_sgns_stats {

@@ -1,0 +1,69 @@
+"""Graph replay and the worker autotuner, through the C ABI.
+
+A pass replayed from the captured HIP graph must match an eager launch bit for
+bit (one worker, so the pass is deterministic), on every replay, with the
+program's own eager work and synchronisations between replays.  The
+autotuner must leave the model as it found it and pick one of its candidate
+worker counts."""
+import numpy as np
+import pytest
+
+import dger_sgns as G
+from corpora import random_corpus
+
+pytestmark = pytest.mark.gpu
+
+
+def opts(graph, autotune=0, precision="fp32"):
+    return G.default_opts(size=64, window=5, negative=5, sample=1e-3, alpha=0.025, iter=1, threads=2,
+                          table_size=1_000_000, precision=precision, debug=0, graph=graph, autotune=autotune)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16-mfma"])
+def test_graph_replays_match_eager(precision):
+    corpus, cn = random_corpus(7, V=2000, n_walks=800, lmin=2, lmax=70)
+    print(f"graph replay {precision} tokens={corpus.size}", flush=True)
+    with G.Trainer(opts(0, precision=precision), cn, corpus) as tr:
+        est, _, _ = tr.train(workers=1)
+        ewih, ewoh = tr.get_model()
+    with G.Trainer(opts(1, precision=precision), cn, corpus) as tr:
+        for rep in range(3):
+            tr.reset_model()
+            gst, _, _ = tr.train(workers=1)
+            gwih, gwoh = tr.get_model()
+            assert gst.steps == est.steps and gst.raw_words == est.raw_words, rep
+            assert np.float32(gst.final_alpha) == np.float32(est.final_alpha), rep
+            assert np.array_equal(gwih, ewih) and np.array_equal(gwoh, ewoh), rep
+            # eager work between replays of the same graph: model reset
+            # (init kernel), model read-back (conversion kernel for bf16
+            # rows) and host round trips
+            _ = tr.vocab()
+
+
+def test_autotune_restores_model():
+    # under 8192 tokens every candidate is one worker (slices of >= 4096
+    # tokens), so the tuned pass is deterministic and must equal a plain one
+    corpus, cn = random_corpus(8, V=1200, n_walks=150, lmin=5, lmax=60)
+    assert corpus.size < 8192
+    print(f"autotune small tokens={corpus.size}", flush=True)
+    with G.Trainer(opts(0), cn, corpus) as tr:
+        pst, _, _ = tr.train(workers=1)
+        pwih, pwoh = tr.get_model()
+    with G.Trainer(opts(1, autotune=1), cn, corpus) as tr:
+        st, _, _ = tr.train()
+        wih, woh = tr.get_model()
+    assert st.workers == 1 and st.steps == pst.steps
+    assert np.array_equal(wih, pwih) and np.array_equal(woh, pwoh)
+
+
+def test_autotune_choice_is_kept():
+    corpus, cn = random_corpus(9, V=3000, n_walks=4000, lmin=5, lmax=80)
+    print(f"autotune tokens={corpus.size}", flush=True)
+    with G.Trainer(opts(1, autotune=1), cn, corpus) as tr:
+        st, _, _ = tr.train()
+        st2, _, _ = tr.train()
+    assert st.workers >= 1 and st2.workers == st.workers
+    with G.Trainer(opts(1), cn, corpus) as tr:
+        ref, _, _ = tr.train(workers=st.workers)
+    # same worker count -> same slices and index streams -> same step count
+    assert ref.steps == st.steps and ref.raw_words == st.raw_words
