@@ -136,9 +136,7 @@ def main() -> int:
         if not args.no_auc:
             tr.reset_model()
             tr.train()
-            rank_id, _ = tr.vocab()
-            wih, _ = tr.get_model()
-            res["auc"] = synth.link_auc(wih, rank_id, test_pairs, test_labels)
+            res["auc"] = tr.link_auc(test_pairs, test_labels)  # scored and ranked on the device
             log(f"[{form}] link-prediction AUC after one pass: {res['auc']:.4f}")
         tr.close()
         results[form] = res

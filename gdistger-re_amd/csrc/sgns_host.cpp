@@ -146,6 +146,7 @@ struct dger_ctx {
   hipGraphExec_t graph_exec = nullptr;
   dger::TrainArgs graph_args{};
   int tuned_workers = 0;  // autotuner's choice (0 = not tuned)
+  DevBuf link_pairs_d, link_scores_d, link_labels_d, link_scratch_d, link_counts_d;
 
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -628,6 +629,45 @@ int dger_ctx_set_model(dger_ctx* c, const float* wih, const float* woh) {
     }
     DGER_HIP(hipStreamSynchronize(c->stream));
   }
+  return DGER_OK;
+}
+
+// Uploads n vertex-id pairs and scores them on the device into c->link_scores_d.
+static int link_scores_device(dger_ctx* c, const int32_t* pairs, uint64_t n) {
+  if (!c) return fail(DGER_E_ARG, "null ctx");
+  if (!pairs && n) return fail(DGER_E_ARG, "null pairs");
+  DGER_HIP(hipSetDevice(c->o.device));
+  if (c->link_pairs_d.ensure(n * 8) || c->link_scores_d.ensure(n * 4))
+    return fail(DGER_E_ALLOC, "link evaluation buffers");
+  DGER_HIP(hipMemcpyAsync(c->link_pairs_d.p, pairs, n * 8, hipMemcpyHostToDevice, c->stream));
+  DGER_HIP(dger::launch_link_scores(c->wih_d.p, c->o.precision != DGER_PRECISION_FP32, c->id_rank_d.as<int32_t>(),
+                                    c->V, c->D, c->link_pairs_d.as<int32_t>(), n, c->link_scores_d.as<float>(),
+                                    c->stream));
+  return DGER_OK;
+}
+
+int dger_ctx_link_scores(dger_ctx* c, const int32_t* pairs, uint64_t n, float* scores) {
+  if (!scores && n) return fail(DGER_E_ARG, "null scores");
+  if (const int rc = link_scores_device(c, pairs, n); rc != DGER_OK) return rc;
+  DGER_HIP(hipMemcpyAsync(scores, c->link_scores_d.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+  DGER_HIP(hipStreamSynchronize(c->stream));
+  return DGER_OK;
+}
+
+int dger_ctx_link_auc(dger_ctx* c, const int32_t* pairs, const int8_t* labels, uint64_t n, double* auc) {
+  if (!auc || (!labels && n)) return fail(DGER_E_ARG, "null pointer");
+  if (n > (uint64_t)INT32_MAX) return fail(DGER_E_UNSUPPORTED, "at most 2^31-1 pairs");
+  if (const int rc = link_scores_device(c, pairs, n); rc != DGER_OK) return rc;
+  if (c->link_labels_d.ensure(n ? n : 1) || c->link_scratch_d.ensure(dger::link_auc_scratch_bytes(n)) ||
+      c->link_counts_d.ensure(3 * 8))
+    return fail(DGER_E_ALLOC, "link evaluation buffers");
+  DGER_HIP(hipMemcpyAsync(c->link_labels_d.p, labels, n, hipMemcpyHostToDevice, c->stream));
+  uint64_t npos = 0, nneg = 0;
+  unsigned long long twice_u = 0;
+  DGER_HIP(dger::launch_link_auc(c->link_scores_d.as<float>(), c->link_labels_d.as<int8_t>(), n,
+                                 c->link_scratch_d.p, c->link_counts_d.as<unsigned long long>(), &npos, &nneg,
+                                 &twice_u, c->stream));
+  *auc = (npos && nneg) ? 0.5 * (double)twice_u / ((double)npos * (double)nneg) : std::nan("");
   return DGER_OK;
 }
 

@@ -71,6 +71,13 @@ hipError_t launch_table_fill(int32_t* table, uint64_t table_size, const int64_t*
                              hipStream_t s);
 hipError_t prepare_train(const TrainArgs& a, int precision);
 hipError_t launch_train(const TrainArgs& a, int precision, hipStream_t s);
+// Link-prediction evaluation (link_eval.hip).
+hipError_t launch_link_scores(const void* wih, bool bf16_rows, const int32_t* id_rank, uint32_t V, int D,
+                              const int32_t* pairs, uint64_t n, float* scores, hipStream_t s);
+size_t link_auc_scratch_bytes(uint64_t n);
+hipError_t launch_link_auc(const float* scores, const int8_t* labels, uint64_t n, void* scratch,
+                           unsigned long long* counts3, uint64_t* npos, uint64_t* nneg,
+                           unsigned long long* twice_u, hipStream_t s);
 // Zeroes n <= 64 counters with a kernel (capturable in a graph).
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s);

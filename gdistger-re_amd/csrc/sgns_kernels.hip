@@ -52,7 +52,10 @@ __device__ __forceinline__ unsigned f32_to_bf16(float f) {
   return u >> 16;
 }
 
-// Row storage traits.  A lane works on pairs of adjacent elements.
+// Row storage traits.  A lane works on pairs of adjacent elements.  Model
+// rows are loaded and stored at agent scope: workers on different XCDs race
+// on the same rows (Hogwild), and plain write-back stores could leave
+// different versions of a row in different XCDs' L2 caches after the pass.
 template <typename R>
 struct Row;
 
@@ -70,8 +73,9 @@ struct Row<float> {
     return widen(load_raw(base, row, D, pr));
   }
   __device__ static void store(void* base, size_t row, int D, int pr, float2 v) {
-    float2* p = reinterpret_cast<float2*>(static_cast<float*>(base) + row * (size_t)D) + pr;
-    *p = v;
+    u64* p = reinterpret_cast<u64*>(static_cast<float*>(base) + row * (size_t)D) + pr;
+    __hip_atomic_store(p, (u64)__float_as_uint(v.x) | ((u64)__float_as_uint(v.y) << 32), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   }
   __device__ static void atomic_add(void* base, size_t row, int D, int pr, float2 v) {
     float* p = static_cast<float*>(base) + row * (size_t)D + 2 * pr;
@@ -95,7 +99,7 @@ struct Row<uint16_t> {
   }
   __device__ static void store(void* base, size_t row, int D, int pr, float2 v) {
     unsigned* p = reinterpret_cast<unsigned*>(static_cast<uint16_t*>(base) + row * (size_t)D) + pr;
-    *p = f32_to_bf16(v.x) | (f32_to_bf16(v.y) << 16);
+    __hip_atomic_store(p, f32_to_bf16(v.x) | (f32_to_bf16(v.y) << 16), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __device__ static void atomic_add(void* base, size_t row, int D, int pr, float2 v) {
     typedef short __attribute__((ext_vector_type(2))) short2v;
@@ -147,6 +151,12 @@ struct Lds {
 };
 
 constexpr int kNone = 1 << 30;
+// Element pairs per lane for row length D, rounded up to 1, 2, 4 or 8 (the
+// instantiated widths; lanes past the row end are masked).
+__host__ __device__ inline int np_class(int D) {
+  const int np = (D / 2 + 63) / 64;
+  return np <= 1 ? 1 : np <= 2 ? 2 : np <= 4 ? 4 : 8;
+}
 constexpr int kMaxWpb = 4;  // workers per workgroup: one wavefront per SIMD, the full register file each
 __host__ __device__ inline size_t exp_lds_bytes() { return ((size_t)kExpTable * 4 + 15) & ~(size_t)15; }
 
@@ -1584,12 +1594,11 @@ template <typename R>
 hipError_t prepare_train_t(const TrainArgs& a, int precision) {
   if (a.wpb < 1 || a.wpb > kMaxWpb) return hipErrorInvalidValue;
   const int lds = (int)(exp_lds_bytes() + (size_t)a.wpb * train_lds_bytes(a.Lmax, a.K, a.W, a.D, precision));
-  switch ((a.D / 2 + 63) / 64) {
+  switch (np_class(a.D)) {
 #define DGER_CASE(X) \
   case X:            \
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train<R, X>), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    DGER_CASE(1) DGER_CASE(2) DGER_CASE(3) DGER_CASE(4)
-    DGER_CASE(5) DGER_CASE(6) DGER_CASE(7) DGER_CASE(8)
+    DGER_CASE(1) DGER_CASE(2) DGER_CASE(4) DGER_CASE(8)
 #undef DGER_CASE
     default: return hipErrorInvalidValue;
   }
@@ -1603,15 +1612,14 @@ template <typename R>
 hipError_t launch_train_t(const TrainArgs& a, hipStream_t st, int precision) {
   if (a.wpb < 1 || a.wpb > kMaxWpb) return hipErrorInvalidValue;
   const size_t lds = exp_lds_bytes() + (size_t)a.wpb * train_lds_bytes(a.Lmax, a.K, a.W, a.D, precision);
-  const int np = (a.D / 2 + 63) / 64;
+  const int np = np_class(a.D);
   dim3 grid((a.workers + a.wpb - 1) / a.wpb), block(64 * a.wpb);
   switch (np) {
 #define DGER_CASE(X)                                                      \
   case X:                                                                 \
     hipLaunchKernelGGL((k_train<R, X>), grid, block, lds, st, a);         \
     break;
-    DGER_CASE(1) DGER_CASE(2) DGER_CASE(3) DGER_CASE(4)
-    DGER_CASE(5) DGER_CASE(6) DGER_CASE(7) DGER_CASE(8)
+    DGER_CASE(1) DGER_CASE(2) DGER_CASE(4) DGER_CASE(8)
 #undef DGER_CASE
     default: return hipErrorInvalidValue;
   }
@@ -1683,15 +1691,13 @@ hipError_t launch_step(int D, int os, int nin, int fin, const int32_t* meta, flo
   const int K = os - 2, W = (nin + 3) / 4 > 0 ? (nin + 3) / 4 : 1;
   size_t off[7];
   const size_t lds = lds_layout(1, K, W, D, off);
-  const int np = (D / 2 + 63) / 64;
-  switch (np) {
+  switch (np_class(D)) {
 #define DGER_CASE(X)                                                                        \
   case X:                                                                                   \
     hipLaunchKernelGGL((k_step<X>), dim3(1), dim3(64), lds, s, D, os, nin, fin, meta, alpha, \
                        exp_table, O, I, corr, md, gin);                                     \
     break;
-    DGER_CASE(1) DGER_CASE(2) DGER_CASE(3) DGER_CASE(4)
-    DGER_CASE(5) DGER_CASE(6) DGER_CASE(7) DGER_CASE(8)
+    DGER_CASE(1) DGER_CASE(2) DGER_CASE(4) DGER_CASE(8)
 #undef DGER_CASE
     default: return hipErrorInvalidValue;
   }

@@ -101,6 +101,8 @@ _SIGS = {
     "dger_dev_download": (C.c_int, [_vp, _vp, C.c_uint64]),
     "dger_dev_zero": (C.c_int, [_vp, C.c_uint64]),
     "dger_dev_copy": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "dger_ctx_link_scores": (C.c_int, [_vp, _vp, C.c_uint64, _vp]),
+    "dger_ctx_link_auc": (C.c_int, [_vp, _vp, _vp, C.c_uint64, C.POINTER(C.c_double)]),
     "dger_device_count": (C.c_int, [C.POINTER(_i32)]),
 }
 EXPORTS = tuple(_SIGS)
@@ -226,6 +228,23 @@ class Trainer:
             raise ValueError("records need an explicit worker count")
         _check(lib().dger_ctx_train(self._h, workers, _ptr(rec), rec_cap, _ptr(ws), C.byref(st)))
         return st, rec, ws
+
+    def link_scores(self, pairs) -> np.ndarray:
+        """Dot-product scores of vertex-id pairs on the device-resident model."""
+        p = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        out = np.empty(p.shape[0], np.float32)
+        _check(lib().dger_ctx_link_scores(self._h, _ptr(p), p.shape[0], _ptr(out)))
+        return out
+
+    def link_auc(self, pairs, labels) -> float:
+        """Link-prediction ROC AUC computed on the device."""
+        p = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        lab = np.ascontiguousarray(labels, np.int8)
+        if lab.size != p.shape[0]:
+            raise ValueError("one label per pair")
+        r = C.c_double()
+        _check(lib().dger_ctx_link_auc(self._h, _ptr(p), _ptr(lab), p.shape[0], C.byref(r)))
+        return float(r.value)
 
     def get_model(self):
         wih = np.empty((self.V, self.D), np.float32)

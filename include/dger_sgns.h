@@ -195,6 +195,16 @@ int dger_dev_free(void *ptr);
 int dger_dev_upload(void *dst, const void *src, uint64_t bytes);
 int dger_dev_download(void *dst, const void *src, uint64_t bytes);
 int dger_dev_zero(void *ptr, uint64_t bytes);
+/* Link prediction on the context's current model (north-star evaluation;
+   held-out split as src/tools/gtrain.cpp:300-343).  pairs: n x 2 vertex ids.
+   dger_ctx_link_scores: scores[i] = dot(Wih[u_i], Wih[v_i]) computed on the
+   device (0 for ids outside the vocabulary).  dger_ctx_link_auc: ROC AUC of
+   those scores against labels (1 = held-out edge, 0 = non-edge) by the
+   Mann-Whitney statistic with ties counting one half, computed on the device;
+   NaN when a class is empty. */
+int dger_ctx_link_scores(dger_ctx *ctx, const int32_t *pairs, uint64_t n, float *scores);
+int dger_ctx_link_auc(dger_ctx *ctx, const int32_t *pairs, const int8_t *labels, uint64_t n, double *auc);
+
 /* Device-to-device copy (also the known-size reference for traffic counters). */
 int dger_dev_copy(void *dst, const void *src, uint64_t bytes);
 int dger_device_count(int32_t *count);
