@@ -147,6 +147,9 @@ struct dger_ctx {
   dger::TrainArgs graph_args{};
   int tuned_workers = 0;  // autotuner's choice (0 = not tuned)
   DevBuf link_pairs_d, link_scores_d, link_labels_d, link_scratch_d, link_counts_d;
+  // FP8 form: master rows of Wih / Woh and the two matrix descriptors.
+  DevBuf wm_d, om_d, desc_d;
+  int n_master = 0;
 
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -256,6 +259,7 @@ int dger_sgns_parse_args(int argc, char** argv, dger_sgns_opts* o) {
     if (!std::strcmp(v, "bf16") || !std::strcmp(v, "1")) o->precision = DGER_PRECISION_BF16;
     else if (!std::strcmp(v, "bf16-mfma") || !std::strcmp(v, "2")) o->precision = DGER_PRECISION_BF16_MFMA;
     else if (!std::strcmp(v, "fp32") || !std::strcmp(v, "0")) o->precision = DGER_PRECISION_FP32;
+    else if (!std::strcmp(v, "fp8") || !std::strcmp(v, "3")) o->precision = DGER_PRECISION_FP8;
     else return fail(DGER_E_ARG, std::string("unknown -precision ") + v);
   }
   if ((v = val("-table-size"))) {
@@ -276,15 +280,58 @@ static int check_opts(const dger_sgns_opts* o) {
   if (o->iter < 1) return fail(DGER_E_ARG, "-iter must be >= 1");
   if (o->table_size < 1) return fail(DGER_E_ARG, "-table-size must be >= 1");
   if (o->precision != DGER_PRECISION_FP32 && o->precision != DGER_PRECISION_BF16 &&
-      o->precision != DGER_PRECISION_BF16_MFMA)
+      o->precision != DGER_PRECISION_BF16_MFMA && o->precision != DGER_PRECISION_FP8)
     return fail(DGER_E_ARG, "unknown precision");
+  // The FP8 form updates whole rows; its shapes must take the register path
+  // (at most 64 contexts, 8 output rows) and rows must keep the scale aligned.
+  if (o->precision == DGER_PRECISION_FP8 && (o->size % 4 != 0 || o->window > 16 || o->negative > 6))
+    return fail(DGER_E_UNSUPPORTED, "-precision fp8 needs -size divisible by 4, -window <= 16, -negative <= 6");
   if (o->workers < 0) return fail(DGER_E_ARG, "-workers must be >= 0");
+  return DGER_OK;
+}
+
+// The model in the context's storage form from FP32 rows (rank order) in
+// tmp_d (Wih when m == 0, Woh when m == 1).
+static int store_model_from_tmp(dger_ctx* c, int m) {
+  const uint64_t n = (uint64_t)c->V * (uint64_t)c->D;
+  DevBuf* dst = m ? &c->woh_d : &c->wih_d;
+  if (c->o.precision == DGER_PRECISION_FP8) {
+    DGER_HIP(dger::launch_fp8_quantize(c->tmp_d.as<float>(), c->desc_d.as<dger::Fp8Mat>() + m, c->V, c->D, c->stream));
+  } else if (c->o.precision != DGER_PRECISION_FP32) {
+    DGER_HIP(dger::launch_f32_to_bf16(c->tmp_d.as<float>(), dst->as<uint16_t>(), n, c->stream));
+  } else {
+    DGER_HIP(hipMemcpyAsync(dst->p, c->tmp_d.p, n * 4, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return DGER_OK;
+}
+
+// FP32 rows (rank order) of Wih (m == 0) or Woh (m == 1) into tmp_d.
+static int load_model_to_tmp(dger_ctx* c, int m) {
+  const uint64_t n = (uint64_t)c->V * (uint64_t)c->D;
+  DGER_HIP(c->tmp_d.ensure(n * 4));
+  DevBuf* src = m ? &c->woh_d : &c->wih_d;
+  if (c->o.precision == DGER_PRECISION_FP8) {
+    DGER_HIP(dger::launch_fp8_dequantize(c->desc_d.as<dger::Fp8Mat>() + m, c->tmp_d.as<float>(), c->V, c->D, c->stream));
+  } else if (c->o.precision != DGER_PRECISION_FP32) {
+    DGER_HIP(dger::launch_bf16_to_f32(src->as<uint16_t>(), c->tmp_d.as<float>(), n, c->stream));
+  } else {
+    DGER_HIP(hipMemcpyAsync(c->tmp_d.p, src->p, n * 4, hipMemcpyDeviceToDevice, c->stream));
+  }
   return DGER_OK;
 }
 
 int dger_ctx_reset_model(dger_ctx* c) {
   if (!c) return fail(DGER_E_ARG, "null ctx");
   const uint64_t n = (uint64_t)c->V * (uint64_t)c->D;
+  if (c->o.precision == DGER_PRECISION_FP8) {
+    DGER_HIP(c->tmp_d.ensure(n * 4));
+    DGER_HIP(dger::launch_init_net(c->tmp_d.as<float>(), n, c->D, c->stream));
+    if (const int rc = store_model_from_tmp(c, 0); rc != DGER_OK) return rc;
+    DGER_HIP(hipMemsetAsync(c->woh_d.p, 0, c->woh_d.bytes, c->stream));
+    DGER_HIP(hipMemsetAsync(c->om_d.p, 0, c->om_d.bytes, c->stream));
+    DGER_HIP(hipStreamSynchronize(c->stream));
+    return DGER_OK;
+  }
   if (c->o.precision != DGER_PRECISION_FP32) {
     DGER_HIP(c->tmp_d.ensure(n * 4));
     DGER_HIP(dger::launch_init_net(c->tmp_d.as<float>(), n, c->D, c->stream));
@@ -354,7 +401,8 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   const size_t rb = row_bytes(opts->precision);
   if (c->id_rank_d.ensure(v_num * 4) || c->keep_d.ensure(v_num * 4) ||
       c->table_d.ensure((size_t)opts->table_size * 4) || c->exp_d.ensure(sizeof(exp_table)) ||
-      c->wih_d.ensure(n * rb) || c->woh_d.ensure(n * rb) ||
+      c->wih_d.ensure(opts->precision == DGER_PRECISION_FP8 ? (size_t)v_num * dger::fp8_row_stride(c->D) : n * rb) ||
+      c->woh_d.ensure(opts->precision == DGER_PRECISION_FP8 ? (size_t)v_num * dger::fp8_row_stride(c->D) : n * rb) ||
       c->counters_d.ensure(dger::kNumCounters * 8) || c->tmp_d.ensure(starts.size() * 8))
     return bail(fail(DGER_E_ALLOC, "device allocation failed"));
   if (hipMemcpyAsync(c->id_rank_d.p, c->id_rank.data(), v_num * 4, hipMemcpyHostToDevice, c->stream) ||
@@ -362,6 +410,18 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
       hipMemcpyAsync(c->exp_d.p, exp_table, sizeof(exp_table), hipMemcpyHostToDevice, c->stream) ||
       hipMemcpyAsync(c->tmp_d.p, starts.data(), starts.size() * 8, hipMemcpyHostToDevice, c->stream))
     return bail(fail(DGER_E_HIP, "upload failed"));
+  if (opts->precision == DGER_PRECISION_FP8) {
+    // FP32 masters for the top 1 % ranks by count (at least one row).
+    c->n_master = (int)std::max<uint64_t>(1, ((uint64_t)v_num + 99) / 100);
+    if (c->wm_d.ensure((size_t)c->n_master * c->D * 4) || c->om_d.ensure((size_t)c->n_master * c->D * 4) ||
+        c->desc_d.ensure(2 * sizeof(dger::Fp8Mat)))
+      return bail(fail(DGER_E_ALLOC, "device allocation failed"));
+    dger::Fp8Mat d[2];
+    d[0] = {c->wih_d.as<uint8_t>(), c->wm_d.as<float>(), c->n_master, c->D};
+    d[1] = {c->woh_d.as<uint8_t>(), c->om_d.as<float>(), c->n_master, c->D};
+    if (hipMemcpy(c->desc_d.p, d, sizeof(d), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(DGER_E_HIP, "upload failed"));
+  }
   if (dger::launch_table_fill(c->table_d.as<int32_t>(), (uint64_t)opts->table_size, c->tmp_d.as<int64_t>(),
                               (int)starts.size(), c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
@@ -426,8 +486,9 @@ static dger::TrainArgs make_args(dger_ctx* c, int workers, int wpb, int32_t* rec
   a.table = c->table_d.as<int32_t>();
   a.table_size = (uint64_t)c->o.table_size;
   a.exp_table = c->exp_d.as<float>();
-  a.wih = c->wih_d.p;
-  a.woh = c->woh_d.p;
+  const bool fp8 = c->o.precision == DGER_PRECISION_FP8;
+  a.wih = fp8 ? (void*)c->desc_d.as<dger::Fp8Mat>() : c->wih_d.p;
+  a.woh = fp8 ? (void*)(c->desc_d.as<dger::Fp8Mat>() + 1) : c->woh_d.p;
   a.fscratch = c->scratch_d.as<float>();
   a.counters = c->counters_d.as<unsigned long long>();
   a.rec = rec;
@@ -443,7 +504,7 @@ static dger::TrainArgs make_args(dger_ctx* c, int workers, int wpb, int32_t* rec
   a.Lmax = c->Lmax;
   a.wpb = wpb;
   a.sample_on = c->o.sample > 0 ? 1 : 0;
-  a.mfma = c->o.precision == DGER_PRECISION_BF16_MFMA ? 1 : 0;
+  a.mfma = (c->o.precision == DGER_PRECISION_BF16_MFMA || fp8) ? 1 : 0;
   a.alpha0 = c->o.alpha;
   a.alpha_den = (float)((uint64_t)c->o.iter * c->train_words + 1);
   return a;
@@ -489,11 +550,13 @@ static int run_pass(dger_ctx* c, const dger::TrainArgs& a, bool use_graph, float
 // workers per CU (one workgroup per CU) and returns the fastest in trained
 // words per second.  The model is saved before and restored after.
 static int autotune_workers(dger_ctx* c, int cus, int wpb_max, int* out) {
-  const uint64_t n = (uint64_t)c->V * (uint64_t)c->D * (c->o.precision == DGER_PRECISION_FP32 ? 4 : 2);
-  DevBuf save_i, save_o;
-  if (save_i.ensure(n) || save_o.ensure(n)) return fail(DGER_E_ALLOC, "autotune: model copy allocation failed");
-  DGER_HIP(hipMemcpyAsync(save_i.p, c->wih_d.p, n, hipMemcpyDeviceToDevice, c->stream));
-  DGER_HIP(hipMemcpyAsync(save_o.p, c->woh_d.p, n, hipMemcpyDeviceToDevice, c->stream));
+  DevBuf* model[4] = {&c->wih_d, &c->woh_d, &c->wm_d, &c->om_d};
+  DevBuf save[4];
+  for (int m = 0; m < 4; m++) {
+    if (!model[m]->p) continue;
+    if (save[m].ensure(model[m]->bytes)) return fail(DGER_E_ALLOC, "autotune: model copy allocation failed");
+    DGER_HIP(hipMemcpyAsync(save[m].p, model[m]->p, model[m]->bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
   const uint64_t sample = std::min<uint64_t>(c->n_tokens, 16ull << 20);
   double best = -1.0;
   int best_w = 0;
@@ -512,8 +575,9 @@ static int autotune_workers(dger_ctx* c, int cus, int wpb_max, int* out) {
       best_w = w;
     }
   }
-  DGER_HIP(hipMemcpyAsync(c->wih_d.p, save_i.p, n, hipMemcpyDeviceToDevice, c->stream));
-  DGER_HIP(hipMemcpyAsync(c->woh_d.p, save_o.p, n, hipMemcpyDeviceToDevice, c->stream));
+  for (int m = 0; m < 4; m++)
+    if (model[m]->p)
+      DGER_HIP(hipMemcpyAsync(model[m]->p, save[m].p, model[m]->bytes, hipMemcpyDeviceToDevice, c->stream));
   DGER_HIP(hipStreamSynchronize(c->stream));
   *out = best_w;
   return DGER_OK;
@@ -601,8 +665,7 @@ int dger_ctx_get_model(dger_ctx* c, float* wih, float* woh) {
   for (int m = 0; m < 2; m++) {
     if (!outs[m]) continue;
     if (c->o.precision != DGER_PRECISION_FP32) {
-      DGER_HIP(c->tmp_d.ensure(n * 4));
-      DGER_HIP(dger::launch_bf16_to_f32(srcs[m]->as<uint16_t>(), c->tmp_d.as<float>(), n, c->stream));
+      if (const int rc = load_model_to_tmp(c, m); rc != DGER_OK) return rc;
       DGER_HIP(hipMemcpyAsync(outs[m], c->tmp_d.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     } else {
       DGER_HIP(hipMemcpyAsync(outs[m], srcs[m]->p, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -623,7 +686,7 @@ int dger_ctx_set_model(dger_ctx* c, const float* wih, const float* woh) {
     if (c->o.precision != DGER_PRECISION_FP32) {
       DGER_HIP(c->tmp_d.ensure(n * 4));
       DGER_HIP(hipMemcpyAsync(c->tmp_d.p, ins[m], n * 4, hipMemcpyHostToDevice, c->stream));
-      DGER_HIP(dger::launch_f32_to_bf16(c->tmp_d.as<float>(), dsts[m]->as<uint16_t>(), n, c->stream));
+      if (const int rc = store_model_from_tmp(c, m); rc != DGER_OK) return rc;
     } else {
       DGER_HIP(hipMemcpyAsync(dsts[m]->p, ins[m], n * 4, hipMemcpyHostToDevice, c->stream));
     }
@@ -640,7 +703,14 @@ static int link_scores_device(dger_ctx* c, const int32_t* pairs, uint64_t n) {
   if (c->link_pairs_d.ensure(n * 8) || c->link_scores_d.ensure(n * 4))
     return fail(DGER_E_ALLOC, "link evaluation buffers");
   DGER_HIP(hipMemcpyAsync(c->link_pairs_d.p, pairs, n * 8, hipMemcpyHostToDevice, c->stream));
-  DGER_HIP(dger::launch_link_scores(c->wih_d.p, c->o.precision != DGER_PRECISION_FP32, c->id_rank_d.as<int32_t>(),
+  const void* rows = c->wih_d.p;
+  bool bf16_rows = c->o.precision != DGER_PRECISION_FP32;
+  if (c->o.precision == DGER_PRECISION_FP8) {  // scored from the decoded FP32 rows
+    if (const int rc = load_model_to_tmp(c, 0); rc != DGER_OK) return rc;
+    rows = c->tmp_d.p;
+    bf16_rows = false;
+  }
+  DGER_HIP(dger::launch_link_scores(rows, bf16_rows, c->id_rank_d.as<int32_t>(),
                                     c->V, c->D, c->link_pairs_d.as<int32_t>(), n, c->link_scores_d.as<float>(),
                                     c->stream));
   return DGER_OK;

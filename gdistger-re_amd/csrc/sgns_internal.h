@@ -71,6 +71,21 @@ hipError_t launch_table_fill(int32_t* table, uint64_t table_size, const int64_t*
                              hipStream_t s);
 hipError_t prepare_train(const TrainArgs& a, int precision);
 hipError_t launch_train(const TrainArgs& a, int precision, hipStream_t s);
+// FP8 form: a model matrix is V rows of D e4m3 codes followed by the row's
+// FP32 scale (row stride D + 4 bytes), plus FP32 master rows for ranks
+// < n_master that take the place of the coded rows.  The trainer gets a
+// device pointer to this descriptor instead of the row array.
+struct Fp8Mat {
+  uint8_t* rows;
+  float* master;
+  int32_t n_master;
+  int32_t D;
+};
+DGER_HD inline size_t fp8_row_stride(int D) { return (size_t)D + 4; }
+// FP32 rows (rank order) <-> an FP8 matrix; quantize rounds to nearest even.
+hipError_t launch_fp8_quantize(const float* src, const Fp8Mat* m, uint32_t V, int D, hipStream_t s);
+hipError_t launch_fp8_dequantize(const Fp8Mat* m, float* dst, uint32_t V, int D, hipStream_t s);
+
 // Link-prediction evaluation (link_eval.hip).
 hipError_t launch_link_scores(const void* wih, bool bf16_rows, const int32_t* id_rank, uint32_t V, int D,
                               const int32_t* pairs, uint64_t n, float* scores, hipStream_t s);

@@ -82,6 +82,21 @@ struct Row<float> {
     unsafeAtomicAdd(p, v.x);
     unsafeAtomicAdd(p + 1, v.y);
   }
+  // Whole-row store / target update (every lane of the wavefront calls them;
+  // lanes past the row end store nothing).
+  template <int NP>
+  __device__ static void store_row(void* base, size_t row, int D, int lane, const float2 (&v)[NP], unsigned) {
+#pragma unroll
+    for (int u = 0; u < NP; u++)
+      if (lane + 64 * u < (D >> 1)) store(base, row, D, lane + 64 * u, v[u]);
+  }
+  template <int NP>
+  __device__ static void add_row(void* base, size_t row, int D, int lane, const float2 (&)[NP],
+                                 const float2 (&d)[NP], unsigned) {
+#pragma unroll
+    for (int u = 0; u < NP; u++)
+      if (lane + 64 * u < (D >> 1)) atomic_add(base, row, D, lane + 64 * u, d[u]);
+  }
 };
 
 template <>
@@ -109,6 +124,109 @@ struct Row<uint16_t> {
     x.y = (short)f32_to_bf16(v.y);
     (void)__builtin_amdgcn_flat_atomic_fadd_v2bf16(p, x);
   }
+  // Whole-row store / target update (every lane of the wavefront calls them;
+  // lanes past the row end store nothing).
+  template <int NP>
+  __device__ static void store_row(void* base, size_t row, int D, int lane, const float2 (&v)[NP], unsigned) {
+#pragma unroll
+    for (int u = 0; u < NP; u++)
+      if (lane + 64 * u < (D >> 1)) store(base, row, D, lane + 64 * u, v[u]);
+  }
+  template <int NP>
+  __device__ static void add_row(void* base, size_t row, int D, int lane, const float2 (&)[NP],
+                                 const float2 (&d)[NP], unsigned) {
+#pragma unroll
+    for (int u = 0; u < NP; u++)
+      if (lane + 64 * u < (D >> 1)) atomic_add(base, row, D, lane + 64 * u, d[u]);
+  }
+};
+
+// FP8 rows (base is a device Fp8Mat): e4m3 codes scaled per row, FP32 master
+// rows for the highest-count ranks.  A row store takes the row's largest
+// magnitude over the wavefront, scales it to 448 (the largest e4m3 value)
+// and rounds every element stochastically (unbiased, so that small updates
+// are not lost to the 3-bit mantissa).  A target update of a coded row is a
+// read-modify-write of the whole row from the worker's staged copy (the
+// reference adds into the shared row without atomics, dsgl.hpp:1352-1356);
+// master rows take FP32 atomic adds.
+__device__ __forceinline__ unsigned hash32(unsigned x) {
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+
+template <>
+struct Row<uint8_t> {
+  __device__ static const Fp8Mat& mat(const void* base) { return *static_cast<const Fp8Mat*>(base); }
+  __device__ static float2 load(const void* base, size_t row, int D, int pr) {
+    const Fp8Mat& m = mat(base);
+    if (row < (size_t)m.n_master) return Row<float>::load(m.master, row, D, pr);
+    const uint8_t* r = m.rows + row * fp8_row_stride(D);
+    const unsigned short q = __hip_atomic_load(const_cast<unsigned short*>(reinterpret_cast<const unsigned short*>(r) + pr),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const float sc = __hip_atomic_load(const_cast<float*>(reinterpret_cast<const float*>(r + D)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+    return make_float2(__builtin_amdgcn_cvt_f32_fp8((int)q, 0) * sc, __builtin_amdgcn_cvt_f32_fp8((int)q, 1) * sc);
+  }
+  // Decoded at load: the prefetch registers hold FP32 pairs.
+  __device__ static u64 load_raw(const void* base, size_t row, int D, int pr) {
+    const float2 v = load(base, row, D, pr);
+    return (u64)__float_as_uint(v.x) | ((u64)__float_as_uint(v.y) << 32);
+  }
+  __device__ static float2 widen(u64 v) { return Row<float>::widen(v); }
+  template <int NP>
+  __device__ static void store_row(void* base, size_t row, int D, int lane, const float2 (&v)[NP], unsigned seed) {
+    const Fp8Mat& m = mat(base);
+    const int DP = D >> 1;
+    if (row < (size_t)m.n_master) {
+#pragma unroll
+      for (int u = 0; u < NP; u++)
+        if (lane + 64 * u < DP) Row<float>::store(m.master, row, D, lane + 64 * u, v[u]);
+      return;
+    }
+    float mx = 0.f;
+#pragma unroll
+    for (int u = 0; u < NP; u++)
+      if (lane + 64 * u < DP) mx = fmaxf(mx, fmaxf(fabsf(v[u].x), fabsf(v[u].y)));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    const float inv = mx > 0.f ? 448.f / mx : 0.f;
+    uint8_t* r = m.rows + row * fp8_row_stride(D);
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      const int pr = lane + 64 * u;
+      if (pr < DP) {
+        const unsigned h = hash32(seed ^ hash32((unsigned)row * 1024u + (unsigned)pr));
+        const float q0 = fminf(fmaxf(v[u].x * inv, -448.f), 448.f);
+        const float q1 = fminf(fmaxf(v[u].y * inv, -448.f), 448.f);
+        int w = __builtin_amdgcn_cvt_sr_fp8_f32(q0, (int)h, 0, 0);
+        w = __builtin_amdgcn_cvt_sr_fp8_f32(q1, (int)hash32(h), w, 1);
+        __hip_atomic_store(reinterpret_cast<unsigned short*>(r) + pr, (unsigned short)(w & 0xFFFF), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (lane == 0)
+      __hip_atomic_store(reinterpret_cast<float*>(r + D), mx > 0.f ? mx / 448.f : 0.f, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+  template <int NP>
+  __device__ static void add_row(void* base, size_t row, int D, int lane, const float2 (&old)[NP],
+                                 const float2 (&d)[NP], unsigned seed) {
+    const Fp8Mat& m = mat(base);
+    if (row < (size_t)m.n_master) {
+#pragma unroll
+      for (int u = 0; u < NP; u++)
+        if (lane + 64 * u < (D >> 1)) Row<float>::atomic_add(m.master, row, D, lane + 64 * u, d[u]);
+      return;
+    }
+    float2 v[NP];
+#pragma unroll
+    for (int u = 0; u < NP; u++) v[u] = make_float2(old[u].x + d[u].x, old[u].y + d[u].y);
+    store_row<NP>(base, row, D, lane, v, seed);
+  }
+  // Element-wise forms are not used for FP8 rows: the host admits the FP8
+  // form only with shapes that take the whole-row paths.
+  __device__ static void store(void*, size_t, int, int, float2) {}
+  __device__ static void atomic_add(void*, size_t, int, int, float2) {}
 };
 
 // Table-quantised sigmoid gradient, dsgl.hpp:1249-1270.  The index is
@@ -710,8 +828,8 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
   const bool can_plan = (K + 2 <= kOC);              // next-step outputs fit one register chunk
   int tab_log2 = 0;  // largest power-of-two table in the staged-output + coefficient areas
   while ((2ull << tab_log2) <= (unsigned long long)KO * OST + cmat_floats(K, W)) tab_log2++;
-  // The BF16 form runs the three contractions on MFMA where the shapes allow.
-  const bool use_mfma = std::is_same<R, uint16_t>::value && a.mfma && mfma_ok(D, K, W);
+  // The BF16-MFMA and FP8 forms run GEMM1 on MFMA where the shapes allow.
+  const bool use_mfma = !std::is_same<R, float>::value && a.mfma && mfma_ok(D, K, W);
   // Output rows beyond the current step's count are read (and their results
   // discarded) by the 8-row GEMM chunks: start them finite.
   for (int x = lane; x < KO * OST; x += 64) s.O[x] = 0.f;
@@ -756,6 +874,11 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
   u64 pre_o[kOC][NP];
   unsigned pre_nb = 0;
 
+  // Random bits for stochastic rounding of FP8 row stores (unused by the
+  // other forms): worker, step and a per-site salt.
+  auto seed_of = [&](unsigned salt) {
+    return hash32((unsigned)wk * 0x9E3779B9u + (unsigned)steps * 0x85EBCA6Bu + salt);
+  };
   // Copies model rows src_rows[ids[k]] for the k < n with first[k] == k to
   // ring entry k of sentence q (k <= W) or to row k of bq, B rows per batch.
   auto gather = [&](const void* src_rows, const int* ids, const int* first, int n, int q, float* bq) {
@@ -845,13 +968,8 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
       }
       if (i0 + B < cnt) fetch(i0 + B);
 #pragma unroll
-      for (int bb = 0; bb < B; bb++) {
-#pragma unroll
-        for (int u = 0; u < NP; u++) {
-          const int pr = lane + 64 * u;
-          if (i0 + bb < cnt && pr < DP) Row<R>::store(dst_rows, (size_t)rk[bb], D, pr, w[bb][u]);
-        }
-      }
+      for (int bb = 0; bb < B; bb++)
+        if (i0 + bb < cnt) Row<R>::template store_row<NP>(dst_rows, (size_t)rk[bb], D, lane, w[bb], seed_of(0x51u));
     }
     wave_sync();
   };
@@ -1197,19 +1315,28 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
     // negative buffer (dsgl.hpp:1358-1366) from their staged values.
     if (fast) {
       const int* oslot = plan_slot(s, K, plan);
+      // Target rows (row 0, then row os-1; when both targets are one rank the
+      // second update starts from the first's result).
+      float2 last_o[NP], last_md[NP];
 #pragma unroll
-      for (int ii = 0; ii < kOC; ii++) {
-        if (ii < os) {
+      for (int ii = 0; ii < kOC; ii++)
+#pragma unroll
+        for (int u = 0; u < NP; u++)
+          if (ii == os - 1) { last_o[u] = o0r[ii][u]; last_md[u] = mdr[ii][u]; }
+      Row<R>::template add_row<NP>(a.woh, (size_t)t1, D, lane, o0r[0], mdr[0], seed_of(0x71u));
+      if (t2 == t1)
+#pragma unroll
+        for (int u = 0; u < NP; u++) last_o[u] = make_float2(o0r[0][u].x + mdr[0][u].x, o0r[0][u].y + mdr[0][u].y);
+      Row<R>::template add_row<NP>(a.woh, (size_t)t2, D, lane, last_o, last_md, seed_of(0x72u));
+#pragma unroll
+      for (int ii = 1; ii < kOC; ii++) {
+        if (ii < os - 1) {
 #pragma unroll
           for (int u = 0; u < NP; u++) {
             const int pr = lane + 64 * u;
-            if (pr < DP) {
-              if (ii == 0) Row<R>::atomic_add(a.woh, (size_t)t1, D, pr, mdr[0][u]);
-              else if (ii == os - 1) Row<R>::atomic_add(a.woh, (size_t)t2, D, pr, mdr[ii][u]);
-              else
-                st2(nbuf + (size_t)oslot[ii] * D + 2 * pr,
-                    make_float2(o0r[ii][u].x + mdr[ii][u].x, o0r[ii][u].y + mdr[ii][u].y));
-            }
+            if (pr < DP)
+              st2(nbuf + (size_t)oslot[ii] * D + 2 * pr,
+                  make_float2(o0r[ii][u].x + mdr[ii][u].x, o0r[ii][u].y + mdr[ii][u].y));
           }
         }
       }
@@ -1340,12 +1467,10 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
         for (int w = wlo; w < Lq; w++) {
           if (pv[w] >= wlo) continue;
           const float* src = ring_row(q, pe[w]);
-          const size_t row = (size_t)tq[w];
+          float2 v[NP];
 #pragma unroll
-          for (int u = 0; u < NP; u++) {
-            const int pr = lane + 64 * u;
-            if (pr < DP) Row<R>::store(a.wih, row, D, pr, ld2(src + 2 * pr));
-          }
+          for (int u = 0; u < NP; u++) v[u] = ld2(src + 2 * (lane + 64 * u));
+          Row<R>::template store_row<NP>(a.wih, (size_t)tq[w], D, lane, v, seed_of(0x33u));
         }
         for (int c0 = 0; c0 < Lq; c0 += lst_cap) {
           const int cnt = compact(c0, min(Lq, c0 + lst_cap), [&](int k) { return sq[k] == k && nx[k] != -2; });
@@ -1468,6 +1593,46 @@ __global__ __launch_bounds__(256) void k_table_fill(int32_t* table, u64 T, const
     }
     table[x] = lo;
   }
+}
+
+// FP32 rows -> FP8 matrix, one wavefront per row (any D up to 64 * 2 * 8):
+// master rows copied, coded rows scaled to 448 and rounded to nearest even.
+__global__ __launch_bounds__(256) void k_fp8_quantize(const float* src, const Fp8Mat* m, uint32_t V, int D) {
+  const u64 row = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= V) return;
+  const float* x = src + row * (u64)D;
+  if (row < (u64)m->n_master) {
+    for (int k = lane; k < D; k += 64) m->master[row * (u64)D + k] = x[k];
+    return;
+  }
+  float mx = 0.f;
+  for (int k = lane; k < D; k += 64) mx = fmaxf(mx, fabsf(x[k]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  const float inv = mx > 0.f ? 448.f / mx : 0.f;
+  uint8_t* r = m->rows + row * fp8_row_stride(D);
+  for (int k = lane; k < D; k += 64) {
+    const float q = fminf(fmaxf(x[k] * inv, -448.f), 448.f);
+    r[k] = (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(q, 0.f, 0, false) & 0xFF);
+  }
+  if (lane == 0) *reinterpret_cast<float*>(r + D) = mx > 0.f ? mx / 448.f : 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_fp8_dequantize(const Fp8Mat* m, float* dst, uint32_t V, int D) {
+  const u64 row = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= V) return;
+  float* y = dst + row * (u64)D;
+  if (row < (u64)m->n_master) {
+    for (int k = lane; k < D; k += 64)
+      y[k] = __hip_atomic_load(m->master + row * (u64)D + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const uint8_t* r = m->rows + row * fp8_row_stride(D);
+  const float sc = __hip_atomic_load(const_cast<float*>(reinterpret_cast<const float*>(r + D)), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+  for (int k = lane; k < D; k += 64) y[k] = __builtin_amdgcn_cvt_f32_fp8((int)r[k], 0) * sc;
 }
 
 __global__ __launch_bounds__(256) void k_f32_to_bf16(const float* src, uint16_t* dst, u64 n) {
@@ -1657,6 +1822,18 @@ hipError_t launch_table_fill(int32_t* table, uint64_t T, const int64_t* starts, 
   return hipGetLastError();
 }
 
+hipError_t launch_fp8_quantize(const float* src, const Fp8Mat* m, uint32_t V, int D, hipStream_t s) {
+  const unsigned blocks = (unsigned)(((uint64_t)V + 3) / 4);
+  hipLaunchKernelGGL(k_fp8_quantize, dim3(blocks), dim3(256), 0, s, src, m, V, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_fp8_dequantize(const Fp8Mat* m, float* dst, uint32_t V, int D, hipStream_t s) {
+  const unsigned blocks = (unsigned)(((uint64_t)V + 3) / 4);
+  hipLaunchKernelGGL(k_fp8_dequantize, dim3(blocks), dim3(256), 0, s, m, dst, V, D);
+  return hipGetLastError();
+}
+
 hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_f32_to_bf16, dim3(4096), dim3(256), 0, s, src, dst, (u64)n);
   return hipGetLastError();
@@ -1668,6 +1845,7 @@ hipError_t launch_bf16_to_f32(const uint16_t* src, float* dst, uint64_t n, hipSt
 }
 
 hipError_t prepare_train(const TrainArgs& a, int precision) {
+  if (precision == DGER_PRECISION_FP8) return prepare_train_t<uint8_t>(a, precision);
   if (precision == DGER_PRECISION_BF16 || precision == DGER_PRECISION_BF16_MFMA)
     return prepare_train_t<uint16_t>(a, precision);
   return prepare_train_t<float>(a, precision);
@@ -1680,6 +1858,7 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
 }
 
 hipError_t launch_train(const TrainArgs& a, int precision, hipStream_t s) {
+  if (precision == DGER_PRECISION_FP8) return launch_train_t<uint8_t>(a, s, precision);
   if (precision == DGER_PRECISION_BF16 || precision == DGER_PRECISION_BF16_MFMA)
     return launch_train_t<uint16_t>(a, s, precision);
   return launch_train_t<float>(a, s, precision);

@@ -30,7 +30,7 @@ DGER_E_ALLOC = -5
 DGER_E_IO = -6
 DGER_E_STATE = -7
 DGER_E_UNSUPPORTED = -8
-PRECISION = {"fp32": 0, "bf16": 1, "bf16-mfma": 2}
+PRECISION = {"fp32": 0, "bf16": 1, "bf16-mfma": 2, "fp8": 3}
 
 
 class LibraryMissing(RuntimeError):

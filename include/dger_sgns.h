@@ -41,6 +41,10 @@ extern "C" {
 #define DGER_PRECISION_FP32 0   /* FP32 rows, FP32 FMA arithmetic */
 #define DGER_PRECISION_BF16 1   /* BF16 rows in HBM, FP32 working buffers, FP32 FMA contractions */
 #define DGER_PRECISION_BF16_MFMA 2 /* BF16 rows, contractions on MFMA (bf16 operands, FP32 accumulate) */
+#define DGER_PRECISION_FP8 3    /* FP8 e4m3 rows with a per-row FP32 scale, FP32 master rows for the
+                                   top 1% ranks by count, stochastic rounding on every row store;
+                                   contractions as BF16_MFMA.  Needs size % 4 == 0, window <= 16,
+                                   negative <= 6. */
 
 /* Options.  The first block mirrors the flags dsgl() parses
  * (include/dsgl.hpp:1496-1535) with the same defaults (dsgl.hpp:67-76);
@@ -70,7 +74,7 @@ typedef struct dger_sgns_opts {
   int32_t workers;         /* -workers     concurrent Hogwild workers; 0 = auto.
                               Worker k trains the corpus slice and LCG seed of the
                               reference's compute thread k+1 under -threads workers+1. */
-  int32_t precision;       /* -precision   fp32 (0), bf16 (1), bf16-mfma (2) */
+  int32_t precision;       /* -precision   fp32 (0), bf16 (1), bf16-mfma (2), fp8 (3) */
   int64_t table_size;      /* -table-size  noise-table slots (1e8, dsgl.hpp:82) */
   int32_t device;          /* -device      HIP device ordinal (0) */
   int32_t graph;           /* -graph       1: the trainer launch (counter reset + kernel) is captured
