@@ -45,12 +45,13 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes_per_word(D: int, K: int, row_bytes: int, S: float = 2.0) -> float:
+def algorithmic_bytes_per_word(D: int, K: int, row_bytes: int, S: float = 2.0, row_extra: int = 0) -> float:
     """Compulsory HBM bytes per trained word: token id, the word's input row
     read at sentence load and written back at sentence end, its target row
     read and written, and K negative rows read and written shared by the S=2
-    targets of a step: 4 + 2R + 2R(1 + K/S)."""
-    R = D * row_bytes
+    targets of a step: 4 + 2R + 2R(1 + K/S), R = D * row_bytes (+ the FP8
+    row scale)."""
+    R = D * row_bytes + row_extra
     return 4 + 2 * R + 2 * R * (1 + K / S)
 
 
@@ -62,7 +63,9 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=21)
     ap.add_argument("--tokens", type=float, default=1e8)
     ap.add_argument("--workers", type=int, default=0, help="0 = library default")
-    ap.add_argument("--forms", default="fp32,bf16,bf16-mfma")
+    ap.add_argument("--forms", default="fp32,bf16,bf16-mfma",
+                    help="comma list of fp32, bf16, bf16-mfma, fp8")
+    ap.add_argument("--dim", type=int, default=128, help="embedding size (256 for the FP8 precision-stress shape)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-auc", action="store_true")
     ap.add_argument("--cpu-tokens", type=float, default=40e6,
@@ -79,7 +82,7 @@ def main() -> int:
     import dger_sgns as G
     from dger_sgns import synth
 
-    D, W, K, sample, alpha = 128, 10, 5, 1e-3, 0.01
+    D, W, K, sample, alpha = args.dim, 10, 5, 1e-3, 0.01
     if args.copy_calibration > 0:
         n = int(args.copy_calibration * (1 << 20))
         src = G.DeviceArray(np.zeros(n, np.uint8))
@@ -141,13 +144,13 @@ def main() -> int:
         tr.close()
         results[form] = res
 
-    bf16_forms = [f for f in ("bf16-mfma", "bf16") if f in results]
+    bf16_forms = [f for f in ("bf16-mfma", "bf16", "fp8") if f in results]
     head = "bf16-mfma" if "bf16-mfma" in results else (bf16_forms[0] if bf16_forms else forms[0])
     h = results[head]
-    row_bytes = 2 if head.startswith("bf16") else 4
-    bpw = algorithmic_bytes_per_word(D, K, row_bytes)
+    row_bytes = {"fp32": 4, "bf16": 2, "bf16-mfma": 2, "fp8": 1}[head]
+    bpw = algorithmic_bytes_per_word(D, K, row_bytes, row_extra=4 if head == "fp8" else 0)
     achieved = h["trained_words_per_step"] * bpw / (h["kernel_ms_avg"] * 1e-3) / 1e9
-    workload = f"rmat-s{args.scale}-deg16-lognormal3.7-{corpus.n_tokens}tok"
+    workload = f"rmat-s{args.scale}-deg16-lognormal3.7-{corpus.n_tokens}tok" + ("" if D == 128 else f"-d{D}")
     peak_tf = BF16_MFMA_PEAK_TFLOPS if head == "bf16-mfma" else FP32_PEAK_TFLOPS
     out = {
         "metric": METRIC,
@@ -160,7 +163,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16" if head.startswith("bf16") else "fp32",
+        "dtype": {"fp32": "fp32", "bf16": "bf16", "bf16-mfma": "bf16", "fp8": "fp8e4m3"}[head],
         "data": "synthetic (R-MAT graph, uniform random walks; seeded)",
         "config": {
             "workload": workload,
@@ -190,7 +193,8 @@ def main() -> int:
     if "fp32" in results and head != "fp32":
         out["bf16_over_fp32"] = results[head]["words_per_sec"] / results["fp32"]["words_per_sec"]
         if "auc" in results["fp32"]:
-            out["auc_gap"] = {f: abs(results[f]["auc"] - results["fp32"]["auc"]) for f in bf16_forms}
+            out["auc_gap"] = {f: abs(results[f]["auc"] - results["fp32"]["auc"])
+                              for f in results if f != "fp32" and "auc" in results[f]}
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(corpus, D, W, K, sample, alpha, int(args.cpu_tokens))
