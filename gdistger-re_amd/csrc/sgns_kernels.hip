@@ -824,7 +824,10 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
   const int MC = 4 * W, RR = 2 * W + 1, RST = ring_stride(D), OST = o_stride(D), KO = ko8(K);
   size_t off_unused[7];
   const Lds s = carve(smem + exp_lds_bytes() + (size_t)wave * lds_layout(Lmax, K, W, D, off_unused), Lmax, K, W, D);
-  constexpr int B = (32 / NP) > 2 ? (32 / NP) : 2;  // rows with loads in flight per batch
+  // Rows with loads in flight per write-back batch, sized to keep the
+  // allocator from spilling (measured per row type at NP = 1).
+  constexpr int B0 = std::is_same<R, float>::value ? 8 : 16;
+  constexpr int B = (B0 / NP) > 2 ? (B0 / NP) : 2;
   const bool can_plan = (K + 2 <= kOC);              // next-step outputs fit one register chunk
   int tab_log2 = 0;  // largest power-of-two table in the staged-output + coefficient areas
   while ((2ull << tab_log2) <= (unsigned long long)KO * OST + cmat_floats(K, W)) tab_log2++;
