@@ -882,6 +882,25 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
   auto seed_of = [&](unsigned salt) {
     return hash32((unsigned)wk * 0x9E3779B9u + (unsigned)steps * 0x85EBCA6Bu + salt);
   };
+  // Raw element pair pr of an output row: negative-buffer row sl (FP32) when
+  // from_nb, else Woh row rk.  Both 4-byte loads are issued unconditionally
+  // from a selected address (no branch: a branch around loads makes the
+  // compiler wait for each one at the join); for bf16 rows the second load
+  // repeats the first.  FP8 rows are decoded at load instead.
+  auto out_pair = [&](bool from_nb, int sl, int rk, int pr) -> u64 {
+    if constexpr (std::is_same<R, uint8_t>::value) {
+      return from_nb ? *reinterpret_cast<const u64*>(nbuf + (size_t)sl * D + 2 * pr)
+                     : Row<R>::load_raw(a.woh, (size_t)rk, D, pr);
+    } else {
+      const char* nrow = reinterpret_cast<const char*>(nbuf + (size_t)sl * D) + 8 * pr;
+      const char* wrow = static_cast<const char*>(a.woh) + (size_t)rk * D * sizeof(R) + 2 * sizeof(R) * pr;
+      const unsigned* p1 = reinterpret_cast<const unsigned*>(from_nb ? nrow : wrow);
+      const unsigned* p2 = reinterpret_cast<const unsigned*>(from_nb ? nrow + 4 : (sizeof(R) == 4 ? wrow + 4 : wrow));
+      const unsigned x1 = __hip_atomic_load(const_cast<unsigned*>(p1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned x2 = __hip_atomic_load(const_cast<unsigned*>(p2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return ((u64)x2 << 32) | x1;
+    }
+  };
   // Copies model rows src_rows[ids[k]] for the k < n with first[k] == k to
   // ring entry k of sentence q (k <= W) or to row k of bq, B rows per batch.
   auto gather = [&](const void* src_rows, const int* ids, const int* first, int n, int q, float* bq) {
@@ -1222,20 +1241,19 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
       }
     } else {
       for (int r0 = 0; r0 < os; r0 += kOC) {
-        float2 v[kOC][NP];
+        u64 v[kOC][NP];
+        bool nb[kOC];
 #pragma unroll
         for (int bb = 0; bb < kOC; bb++) {
           const int r = r0 + bb;
+          nb[bb] = false;
           if (r < os) {
+            const bool tgt = (r == 0) || (r == os - 1);
+            const int sl = tgt ? 0 : plan_slot(s, K, plan)[r];
+            const int rk = (r == 0) ? t1 : (r == os - 1) ? t2 : plan_idx(s, K, plan)[r];
+            nb[bb] = !tgt && nsnap(sl);
 #pragma unroll
-            for (int u = 0; u < NP; u++) {
-              const int pr = lane + 64 * u;
-              if (pr < DP)
-                v[bb][u] = (r == 0) ? Row<R>::load(a.woh, (size_t)t1, D, pr)
-                         : (r == os - 1) ? Row<R>::load(a.woh, (size_t)t2, D, pr)
-                         : nsnap(plan_slot(s, K, plan)[r]) ? ld2(nbuf + (size_t)plan_slot(s, K, plan)[r] * D + 2 * pr)
-                                                            : Row<R>::load(a.woh, (size_t)plan_idx(s, K, plan)[r], D, pr);
-            }
+            for (int u = 0; u < NP; u++) v[bb][u] = out_pair(nb[bb], sl, rk, min(lane + 64 * u, DP - 1));
           }
         }
 #pragma unroll
@@ -1245,7 +1263,8 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
 #pragma unroll
             for (int u = 0; u < NP; u++) {
               const int pr = lane + 64 * u;
-              if (pr < DP) st2(s.O + r * OST + 2 * pr, v[bb][u]);
+              if (pr < DP)
+                st2(s.O + r * OST + 2 * pr, nb[bb] ? Row<float>::widen(v[bb][u]) : Row<R>::widen(v[bb][u]));
             }
           }
         }
@@ -1367,20 +1386,16 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
       const int lsl = neg_row ? plan_slot(s, K, np)[lane] : 0;
       const int lrk = (lane == 0) ? t1n : (lane == osn - 1) ? t2n : (neg_row ? plan_idx(s, K, np)[lane] : 0);
       pre_nb = (unsigned)__ballot(neg_row && nsnap(lsl));
+      // Every row and lane loads (rows past osn re-read the first target,
+      // lanes past the row end its last pair): no branch around the loads,
+      // so none of them is waited for here.
 #pragma unroll
       for (int i = 0; i < kOC; i++) {
-        if (i < osn) {
-          const int sl = __builtin_amdgcn_readlane(lsl, i);
-          const int rk = __builtin_amdgcn_readlane(lrk, i);
-          const bool from_nb = (pre_nb >> i) & 1u;
+        const int sl = __builtin_amdgcn_readlane(lsl, i);
+        const int rk = (i < osn) ? __builtin_amdgcn_readlane(lrk, i) : t1n;
+        const bool from_nb = (pre_nb >> i) & 1u;
 #pragma unroll
-          for (int u = 0; u < NP; u++) {
-            const int pr = lane + 64 * u;
-            if (pr < DP)
-              pre_o[i][u] = from_nb ? *reinterpret_cast<const u64*>(nbuf + (size_t)sl * D + 2 * pr)
-                                    : Row<R>::load_raw(a.woh, (size_t)rk, D, pr);
-          }
-        }
+        for (int u = 0; u < NP; u++) pre_o[i][u] = out_pair(from_nb, sl, rk, min(lane + 64 * u, DP - 1));
       }
       pre_os = osn;
       if (neg_row) atomicOr(&s.nflag[lsl >> 5], 1u << (lsl & 31));
