@@ -825,8 +825,9 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
   size_t off_unused[7];
   const Lds s = carve(smem + exp_lds_bytes() + (size_t)wave * lds_layout(Lmax, K, W, D, off_unused), Lmax, K, W, D);
   // Rows with loads in flight per write-back batch, sized to keep the
-  // allocator from spilling (measured per row type at NP = 1).
-  constexpr int B0 = std::is_same<R, float>::value ? 8 : 16;
+  // allocator from spilling (measured per row type at NP = 1: 4 FP32 rows,
+  // 16 bf16 / fp8 rows).
+  constexpr int B0 = std::is_same<R, float>::value ? 4 : 16;
   constexpr int B = (B0 / NP) > 2 ? (B0 / NP) : 2;
   const bool can_plan = (K + 2 <= kOC);              // next-step outputs fit one register chunk
   int tab_log2 = 0;  // largest power-of-two table in the staged-output + coefficient areas
@@ -914,7 +915,7 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
 #pragma unroll
         for (int u = 0; u < NP; u++) {
           const int pr = lane + 64 * u;
-          if (on && pr < DP) v[b][u] = Row<R>::load(src_rows, row, D, pr);
+          v[b][u] = Row<R>::load(src_rows, row, D, min(pr, DP - 1));  // row 0 when off: unused
         }
       }
       // LDS and global destinations in separate loops, so that no store
@@ -974,7 +975,7 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
 #pragma unroll
         for (int u = 0; u < NP; u++) {
           const int pr = lane + 64 * u;
-          if (i < cnt && pr < DP) v[bb][u] = ld2(src + (size_t)k * D + 2 * pr);
+          v[bb][u] = ld2(src + (size_t)k * D + 2 * min(pr, DP - 1));  // row 0 when i >= cnt: unused
         }
       }
     };
@@ -1209,7 +1210,7 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
 #pragma unroll
       for (int u = 0; u < NP; u++) {
         const int pr = lane + 64 * u;
-        pin[q][u] = (pin_on[q] && pr < DP) ? ld2(src + 2 * pr) : make_float2(0.f, 0.f);
+        pin[q][u] = ld2(src + 2 * min(pr, DP - 1));  // always a valid scratch row; used when pin_on
       }
     }
     wave_sync();
