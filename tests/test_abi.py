@@ -51,6 +51,16 @@ def test_parse_args_flags():
     assert np.float32(o.alpha) == np.float32(0.01)
 
 
+def test_parse_args_gpu_options():
+    o = G.parse_args(["prog", "-precision", "fp8", "-graph", "0", "-autotune", "1", "-table-size", "1000"])
+    assert o.precision == G.PRECISION["fp8"] == 3 and o.graph == 0 and o.autotune == 1 and o.table_size == 1000
+    d = G.default_opts()
+    assert d.graph == 1 and d.autotune == 0 and d.precision == 0
+    for name, code in G.PRECISION.items():
+        assert G.parse_args(["prog", "-precision", name]).precision == code
+        assert G.parse_args(["prog", "-precision", str(code)]).precision == code
+
+
 def test_parse_args_missing_value():
     with pytest.raises(G.DgerError) as e:
         G.parse_args(["prog", "-window"])
@@ -64,7 +74,12 @@ def test_bad_options_rejected_before_device():
     cn = np.ones(10, np.int32)
     for kw, code in [(dict(size=7), G.DGER_E_UNSUPPORTED), (dict(size=2048), G.DGER_E_UNSUPPORTED),
                      (dict(window=0), G.DGER_E_UNSUPPORTED), (dict(negative=-1), G.DGER_E_UNSUPPORTED),
-                     (dict(iter=0), G.DGER_E_ARG), (dict(table_size=0), G.DGER_E_ARG)]:
+                     (dict(iter=0), G.DGER_E_ARG), (dict(table_size=0), G.DGER_E_ARG),
+                     # FP8 rows need 4-byte aligned scales and the whole-row step paths
+                     (dict(size=66, precision="fp8"), G.DGER_E_UNSUPPORTED),
+                     (dict(size=64, window=17, precision="fp8"), G.DGER_E_UNSUPPORTED),
+                     (dict(size=64, negative=7, precision="fp8"), G.DGER_E_UNSUPPORTED),
+                     (dict(precision=9), G.DGER_E_ARG)]:
         with pytest.raises(G.DgerError) as e:
             G.Trainer(G.default_opts(**kw), cn)
         assert e.value.code == code, kw
