@@ -128,6 +128,8 @@ size_t row_bytes(int precision) { return precision == DGER_PRECISION_FP32 ? 4 : 
 
 }  // namespace
 
+int dger::host_fail(int code, const std::string& msg) { return fail(code, msg); }
+
 struct dger_ctx {
   dger_sgns_opts o;
   hipStream_t stream = nullptr;

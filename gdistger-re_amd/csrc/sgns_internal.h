@@ -5,6 +5,8 @@
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace dger {
 
 constexpr unsigned long long kLcgMul = 25214903917ULL;  // dsgl.hpp:453
@@ -85,6 +87,10 @@ DGER_HD inline size_t fp8_row_stride(int D) { return (size_t)D + 4; }
 // FP32 rows (rank order) <-> an FP8 matrix; quantize rounds to nearest even.
 hipError_t launch_fp8_quantize(const float* src, const Fp8Mat* m, uint32_t V, int D, hipStream_t s);
 hipError_t launch_fp8_dequantize(const Fp8Mat* m, float* dst, uint32_t V, int D, hipStream_t s);
+
+// Records msg as the calling thread's last error and returns code (the
+// dger_last_error() of every entry point in the library).
+int host_fail(int code, const std::string& msg);
 
 // Link-prediction evaluation (link_eval.hip).
 hipError_t launch_link_scores(const void* wih, bool bf16_rows, const int32_t* id_rank, uint32_t V, int D,

@@ -70,6 +70,24 @@ class Stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class WalkOpts(C.Structure):
+    _fields_ = [
+        ("n_walkers", C.c_uint64), ("first_walker", C.c_uint64), ("min_length", C.c_uint32),
+        ("max_length", C.c_uint32), ("r_threshold", C.c_double), ("seed", C.c_uint64),
+        ("device", C.c_int32), ("chunk_walkers", C.c_uint32),
+    ]
+
+
+class WalkStats(C.Structure):
+    _fields_ = [
+        ("walks", C.c_uint64), ("tokens", C.c_uint64), ("moves", C.c_uint64), ("traces", C.c_uint64),
+        ("capped", C.c_uint64), ("walk_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _vp = C.c_void_p
 _i32 = C.c_int32
 _SIGS = {
@@ -104,6 +122,10 @@ _SIGS = {
     "dger_ctx_link_scores": (C.c_int, [_vp, _vp, C.c_uint64, _vp]),
     "dger_ctx_link_auc": (C.c_int, [_vp, _vp, _vp, C.c_uint64, C.POINTER(C.c_double)]),
     "dger_device_count": (C.c_int, [C.POINTER(_i32)]),
+    "dger_walk_default_opts": (None, [C.POINTER(WalkOpts)]),
+    "dger_walk_common_neighbors": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _i32]),
+    "dger_walk_run": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.POINTER(WalkOpts), _vp, C.c_uint64,
+                                C.POINTER(C.c_uint64), _vp, C.POINTER(WalkStats)]),
 }
 EXPORTS = tuple(_SIGS)
 
@@ -173,6 +195,40 @@ def default_opts(**kw) -> Opts:
 
 def _i32arr(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def walk_default_opts(**kw) -> WalkOpts:
+    o = WalkOpts()
+    lib().dger_walk_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def common_neighbors(rowptr, col, device: int = 0) -> np.ndarray:
+    """|N(u) ∩ N(col[e])| per edge (dger_walk_common_neighbors); rows sorted."""
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    col = _i32arr(col)
+    out = np.zeros(col.size, np.int32)
+    _check(lib().dger_walk_common_neighbors(_ptr(rowptr), _ptr(col), rowptr.size - 1, _ptr(out), device))
+    return out
+
+
+def walk(rowptr, col, cnb, opts: WalkOpts):
+    """Information-oriented walks on the GPU (dger_walk_run).  Returns the
+    corpus (vertex ids, -1 after every walk), vertex_cn and the stats."""
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    col = _i32arr(col)
+    cnb = _i32arr(cnb)
+    V = rowptr.size - 1
+    cap = int(opts.n_walkers) * (int(opts.max_length) + 1)
+    corpus = np.empty(max(cap, 1), np.int32)
+    vcn = np.zeros(V, np.int32)
+    n = C.c_uint64(0)
+    st = WalkStats()
+    _check(lib().dger_walk_run(_ptr(rowptr), _ptr(col), _ptr(cnb), V, C.byref(opts), _ptr(corpus), cap,
+                               C.byref(n), _ptr(vcn), C.byref(st)))
+    return corpus[:n.value].copy(), vcn, st
 
 
 class Trainer:

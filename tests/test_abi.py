@@ -12,12 +12,14 @@ import dger_sgns as G
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "dger_sgns.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("dger_sgns.h", "dger_walk.h")]
 
 
 def header_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = set(re.findall(r"\b(dger_[a-z0-9_]+)\s*\(", text))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(dger_[a-z0-9_]+)\s*\(", text))
     return names
 
 
@@ -106,6 +108,24 @@ def test_no_device_reports_hip_error():
     with pytest.raises(G.DgerError) as e:
         G.Trainer(G.default_opts(size=8), np.ones(10, np.int32))
     assert e.value.code in (G.DGER_E_HIP, G.DGER_E_ARG)
+
+
+def test_walk_bad_graphs_rejected_before_device():
+    rowptr = np.array([0, 1, 2], np.int64)
+    with pytest.raises(G.DgerError) as e:      # neighbour id out of range
+        G.walk(rowptr, np.array([1, 5], np.int32), np.zeros(2, np.int32), G.walk_default_opts(n_walkers=2))
+    assert e.value.code == G.DGER_E_ARG
+    with pytest.raises(G.DgerError) as e:      # decreasing rowptr
+        G.walk(np.array([0, 2, 1], np.int64), np.array([1, 0], np.int32), np.zeros(2, np.int32),
+               G.walk_default_opts(n_walkers=2))
+    assert e.value.code == G.DGER_E_ARG
+    with pytest.raises(G.DgerError) as e:      # max_length outside [2, 2^20]
+        G.walk(rowptr, np.array([1, 0], np.int32), np.zeros(2, np.int32),
+               G.walk_default_opts(n_walkers=2, max_length=1))
+    assert e.value.code == G.DGER_E_UNSUPPORTED
+    with pytest.raises(G.DgerError) as e:      # unsorted row for common neighbours
+        G.common_neighbors(np.array([0, 2, 3, 4], np.int64), np.array([2, 1, 0, 0], np.int32))
+    assert e.value.code == G.DGER_E_ARG
 
 
 def test_record_stride():

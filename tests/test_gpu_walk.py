@@ -1,0 +1,94 @@
+"""GPU corpus producer (dger_walk_run, dger_walk_common_neighbors) against
+the walk oracle, through the C ABI.
+
+The walker's index work is exact: the corpus, vertex_cn and the move / trace
+/ cut counters must equal the oracle's bit for bit (same splitmix64 stream;
+the double arithmetic of the correlation is evaluated as written on both
+sides).  Cases: an ER graph, R-MAT graphs with isolated vertices and hubs,
+several launches per call (small chunks), a walker-id offset, and the
+reference's stop rules (a degree-2 vertex whose first neighbour is itself,
+self loops on the first move, the max_length cut).  Common-neighbour counts
+are checked against set intersections."""
+import numpy as np
+import pytest
+
+import dger_sgns as G
+from dger_sgns import synth
+from oracle import walk_oracle as WO
+
+pytestmark = pytest.mark.gpu
+
+
+def sorted_csr(n, u, v):
+    rowptr, col = synth.csr(n, u, v)
+    for x in range(n):
+        col[rowptr[x]:rowptr[x + 1]].sort()
+    return rowptr, col
+
+
+def common_neighbors_np(rowptr, col):
+    nb = [set(col[rowptr[u]:rowptr[u + 1]].tolist()) for u in range(rowptr.size - 1)]
+    out = np.empty(col.size, np.int32)
+    for u in range(rowptr.size - 1):
+        for e in range(rowptr[u], rowptr[u + 1]):
+            out[e] = len(nb[u] & nb[col[e]])
+    return out
+
+
+def check_same(rowptr, col, cnb, **kw):
+    o = G.walk_default_opts(**kw)
+    corpus, vcn, st = G.walk(rowptr, col, cnb, o)
+    okw = {k: kw[k] for k in ("first", "min_length", "max_length", "r_threshold", "seed") if k in kw}
+    if "first_walker" in kw:
+        okw["first"] = kw["first_walker"]
+    rc, rv, rs = WO.walk(rowptr, col, cnb, n_walkers=kw["n_walkers"], **okw)
+    print(f"walk V={rowptr.size - 1} walkers={kw['n_walkers']} tokens={st.tokens} oracle={rs}", flush=True)
+    assert corpus.size == rc.size and np.array_equal(corpus, rc)
+    assert np.array_equal(vcn, rv)
+    assert (st.walks, st.tokens, st.moves, st.traces, st.capped) == \
+        (rs["walks"], rs["tokens"], rs["moves"], rs["traces"], rs["capped"])
+    return corpus, st
+
+
+def test_common_neighbors_rmat():
+    u, v, n = synth.rmat_graph(10, 6000, 0.57, 0.19, 0.19, seed=2)
+    rowptr, col = sorted_csr(n, u, v)
+    print(f"common neighbours V={n} nnz={col.size}", flush=True)
+    got = G.common_neighbors(rowptr, col)
+    assert np.array_equal(got, common_neighbors_np(rowptr, col))
+
+
+def test_walk_matches_oracle_er():
+    n = 2000
+    u, v = synth.er_graph(n, 12000, seed=4)
+    rowptr, col = sorted_csr(n, u, v)
+    cnb = G.common_neighbors(rowptr, col)
+    check_same(rowptr, col, cnb, n_walkers=5 * n, seed=3)
+    check_same(rowptr, col, cnb, n_walkers=3000, first_walker=777, min_length=5, seed=9)
+
+
+def test_walk_matches_oracle_rmat_chunks():
+    u, v, n = synth.rmat_graph(14, 16 * (1 << 14), 0.57, 0.19, 0.19, seed=5)
+    rowptr, col = sorted_csr(n, u, v)
+    assert (np.diff(rowptr) == 0).any()          # isolated vertices: one-vertex walks
+    cnb = G.common_neighbors(rowptr, col)
+    corpus, st = check_same(rowptr, col, cnb, n_walkers=2 * n, chunk_walkers=5000, seed=1)
+    assert st.capped == 0
+
+
+def test_stop_rules_and_cut():
+    # 0: [0, 1] (degree 2, first neighbour itself); 1: [0, 2, 3]; 2: [1, 3];
+    # 3: [1, 2, 4]; 4: [4] (a lone self loop); 5: no neighbours
+    rows = [[0, 1], [0, 2, 3], [1, 3], [1, 2, 4], [4], []]
+    rowptr = np.zeros(len(rows) + 1, np.int64)
+    rowptr[1:] = np.cumsum([len(r) for r in rows])
+    col = np.array([x for r in rows for x in r], np.int32)
+    cnb = common_neighbors_np(rowptr, col)
+    corpus, st = check_same(rowptr, col, cnb, n_walkers=600, seed=5)
+    walks = np.split(corpus, np.nonzero(corpus == -1)[0] + 1)[:-1]
+    assert all(w.size == 2 for w in walks[0::6])   # start 0: stops at once (length 1 + separator)
+    assert all(w.size == 2 for w in walks[4::6])   # lone self loop: gives up
+    assert all(w.size == 2 for w in walks[5::6])   # no neighbours
+    # a cut well below the natural length
+    _, st = check_same(rowptr, col, cnb, n_walkers=600, max_length=8, min_length=20, seed=5)
+    assert st.capped > 0
