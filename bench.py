@@ -20,6 +20,13 @@ of the same corpus).
 --copy-calibration MIB runs one device-to-device copy of that size before the
 workload: the known-size launch that scales the FETCH_SIZE / WRITE_SIZE
 counters of a PMC run.
+
+--walker measures the GPU corpus producer instead and prints its own line:
+information-oriented walks (5 walkers per vertex, min_length 20,
+R = 0.999) on the same R-MAT 2^21 graph (sorted rows, common-neighbour
+counts computed on the GPU first), walk tokens/s from HIP-event time of the
+walk kernels, with the walk oracle (one host thread) on a bounded sample of
+the walkers as the CPU baseline.
 """
 from __future__ import annotations
 
@@ -72,12 +79,15 @@ def main() -> int:
                     help="prefix of the corpus the CPU baseline trains on")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="MiB of device-to-device copy to run first (PMC calibration)")
+    ap.add_argument("--walker", action="store_true", help="measure the GPU corpus producer")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         return 2
+    if args.walker:
+        return walker_bench(args)
 
     import dger_sgns as G
     from dger_sgns import synth
@@ -239,6 +249,61 @@ def cpu_baseline(corpus, D, W, K, sample, alpha, n_tokens):
         "sample": f"first {prefix.size} tokens of the same corpus, same vocabulary and options, "
                   f"{cores} OpenMP Hogwild workers, {st.train_seconds:.1f}s of training",
     }
+
+
+def walker_bench(args) -> int:
+    import dger_sgns as G
+    from dger_sgns import synth
+
+    t0 = time.time()
+    u, v, n = synth.rmat_graph(args.scale, 16 * (1 << args.scale), 0.57, 0.19, 0.19, seed=3)
+    rowptr, col = synth.csr(n, u, v)
+    col = col[np.lexsort((col, np.repeat(np.arange(n), np.diff(rowptr))))]
+    log(f"graph: V={n} directed edges={col.size} ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    cnb = G.common_neighbors(rowptr, col)
+    log(f"common neighbours on the GPU: {time.time() - t0:.2f}s wall (incl. transfers)")
+    opts = G.walk_default_opts(n_walkers=5 * n, seed=1)
+    for i in range(args.warmup):
+        G.walk_device(rowptr, col, cnb, opts)[0].close()
+    runs = []
+    for i in range(args.steps):
+        t0 = time.time()
+        wc, vcn, st = G.walk_device(rowptr, col, cnb, opts)
+        wc.close()
+        runs.append((st.walk_ms, time.time() - t0, st))
+        log(f"walk pass {i}: {st.tokens} tokens, kernel {st.walk_ms:.2f} ms, call {runs[-1][1]:.2f}s")
+    ms = sum(r[0] for r in runs) / len(runs)
+    st = runs[-1][2]
+    tok_s = st.tokens / (ms / 1e3)
+    # algorithmic bytes of one move: rowptr pair of the current and of the
+    # drawn vertex (2 x 16 B), the neighbour id and its common-neighbour count
+    # (8 B), the path word written (4 B); a rejected first-move self loop
+    # reads the same words.
+    moves_bytes = st.moves * 44 + st.walks * 20
+    out = {
+        "metric": "walk tokens/sec (information-oriented walks, corpus producer)",
+        "value": tok_s, "unit": "tokens/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"R-MAT 2^{args.scale} (avg degree 16, abc 0.57/0.19/0.19, seed 3), "
+                               f"5 walkers per vertex, min_length 20, R 0.999",
+                   "walks": st.walks, "tokens": st.tokens, "moves": st.moves, "traces": st.traces},
+        "roofline": {"bound": "hbm", "achieved": moves_bytes / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": moves_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, ROOT)
+        from oracle import walk_oracle as WO
+        sample = min(opts.n_walkers, 8_000_000)
+        t0 = time.time()
+        _, _, cst = WO.walk(rowptr, col, cnb, n_walkers=sample, seed=1)
+        el = time.time() - t0
+        out["cpu_baseline"] = {"value": cst["tokens"] / el, "unit": "tokens/s", "cores": 1, "kind": "port",
+                               "sample": f"first {sample} walkers of the same workload, walk oracle, "
+                                         f"one host thread, {el:.1f}s"}
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 if __name__ == "__main__":

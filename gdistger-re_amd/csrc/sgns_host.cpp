@@ -452,6 +452,20 @@ int dger_ctx_set_corpus(dger_ctx* c, const int32_t* corpus, uint64_t n_tokens) {
   return DGER_OK;
 }
 
+int dger_ctx_set_corpus_walks(dger_ctx* c, const dger_walk_corpus* w) {
+  if (!c || !w) return fail(DGER_E_ARG, "null argument");
+  if (w->device != c->o.device) return fail(DGER_E_ARG, "walks are on another device");
+  if (w->v_num > c->V) return fail(DGER_E_CORPUS, "walks use ids beyond the context's v_num");
+  const uint64_t n_tokens = w->n_tokens;
+  c->Lmax = std::max(2, std::min((int)std::min<uint32_t>(w->max_walk, 1u << 30), dger::kMaxSentence));
+  if (c->corpus_d.ensure((n_tokens ? n_tokens : 1) * 4)) return fail(DGER_E_ALLOC, "corpus allocation failed");
+  if (n_tokens) DGER_HIP(hipMemcpyAsync(c->corpus_d.p, w->d, n_tokens * 4, hipMemcpyDeviceToDevice, c->stream));
+  DGER_HIP(hipStreamSynchronize(c->stream));
+  c->n_tokens = n_tokens;
+  c->have_corpus = true;
+  return DGER_OK;
+}
+
 // Default worker count: as many workgroups per CU as its LDS holds, at most
 // one wavefront per SIMD (the trainer uses more than 256 registers).
 static int auto_workers(dger_ctx* c, size_t block_lds, int wpb) {

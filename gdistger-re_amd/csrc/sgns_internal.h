@@ -7,6 +7,16 @@
 
 #include <string>
 
+// Walks kept in HBM by the corpus producer (walk.hip): device pointer,
+// token count (separators included), longest walk, vertex count, device.
+struct dger_walk_corpus {
+  int32_t* d = nullptr;
+  uint64_t n_tokens = 0;
+  uint32_t max_walk = 0;
+  uint32_t v_num = 0;
+  int device = 0;
+};
+
 namespace dger {
 
 constexpr unsigned long long kLcgMul = 25214903917ULL;  // dsgl.hpp:453

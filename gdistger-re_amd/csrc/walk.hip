@@ -41,7 +41,10 @@
 
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
+#include <new>
+#include <utility>
 #include <string>
 
 #include "dger_sgns.h"
@@ -173,20 +176,30 @@ __global__ __launch_bounds__(256) void k_walk(const int64_t* __restrict__ rowptr
 }
 
 // Walk t's vertices go to corpus[off[t] ...], then -1; off is the inclusive
-// sum of len + 1, so walk t starts at off[t - 1].
+// sum of len + 1, so walk t starts at off[t - 1].  max_walk keeps the
+// longest walk (one atomic per wavefront).
 __global__ __launch_bounds__(256) void k_walk_pack(const int32_t* __restrict__ path, const u64* __restrict__ len1,
                                                    const u64* __restrict__ off_incl, u64 n, int32_t* __restrict__ corpus,
-                                                   int32_t* __restrict__ vertex_cn) {
+                                                   int32_t* __restrict__ vertex_cn, u64* __restrict__ max_walk) {
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const u64 o = t == 0 ? 0 : off_incl[t - 1];
-  const uint32_t len = (uint32_t)(len1[t] - 1);
-  for (uint32_t i = 0; i < len; ++i) {
-    const int32_t v = path[(u64)i * n + t];
-    corpus[o + i] = v;
-    atomicAdd(&vertex_cn[v], 1);
+  u64 longest = 0;
+  if (t < n) {
+    const u64 o = t == 0 ? 0 : off_incl[t - 1];
+    const uint32_t len = (uint32_t)(len1[t] - 1);
+    for (uint32_t i = 0; i < len; ++i) {
+      const int32_t v = path[(u64)i * n + t];
+      corpus[o + i] = v;
+      atomicAdd(&vertex_cn[v], 1);
+    }
+    corpus[o + len] = -1;
+    longest = len;
   }
-  corpus[o + len] = -1;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const u64 x = __shfl_xor(longest, off, 64);
+    longest = x > longest ? x : longest;
+  }
+  if ((threadIdx.x & 63) == 0 && longest) atomicMax(max_walk, longest);
 }
 
 // |N(u) ∩ N(v)| for every directed edge (u, v = col[e]); rows sorted
@@ -299,37 +312,44 @@ int dger_walk_common_neighbors(const int64_t* rowptr, const int32_t* col, uint32
   return DGER_OK;
 }
 
-int dger_walk_run(const int64_t* rowptr, const int32_t* col, const int32_t* cnb, uint32_t v_num,
-                  const dger_walk_opts* o, int32_t* corpus, uint64_t corpus_cap, uint64_t* corpus_len,
-                  int32_t* vertex_cn, dger_walk_stats* st) {
-  if (!o || !corpus_len || !vertex_cn) return host_fail(DGER_E_ARG, "null argument");
+int dger_walk_run_device(const int64_t* rowptr, const int32_t* col, const int32_t* cnb, uint32_t v_num,
+                         const dger_walk_opts* o, dger_walk_corpus** out, int32_t* vertex_cn, dger_walk_stats* st) {
+  if (!o || !out || !vertex_cn) return host_fail(DGER_E_ARG, "null argument");
+  *out = nullptr;
   if (int rc = check_graph(rowptr, col, v_num, false)) return rc;
   const uint64_t nnz = (uint64_t)rowptr[v_num];
   if (nnz && !cnb) return host_fail(DGER_E_ARG, "null cnb");
   if (o->min_length < 1) return host_fail(DGER_E_ARG, "min_length must be >= 1");
   if (o->max_length < 2 || o->max_length > (1u << 20)) return host_fail(DGER_E_UNSUPPORTED, "max_length must be in [2, 2^20]");
   if (!(o->r_threshold >= 0.0 && o->r_threshold <= 1.0)) return host_fail(DGER_E_ARG, "r_threshold must be in [0, 1]");
-  if (o->n_walkers && !corpus && corpus_cap) return host_fail(DGER_E_ARG, "null corpus");
   int ndev = 0;
   WALK_HIP(hipGetDeviceCount(&ndev));
   if (o->device < 0 || o->device >= ndev) return host_fail(DGER_E_ARG, "device ordinal out of range");
   WALK_HIP(hipSetDevice(o->device));
   const uint64_t total = o->n_walkers;
-  // Paths of one chunk take chunk * max_length * 4 bytes; default about 2 GiB.
-  uint64_t chunk = o->chunk_walkers ? o->chunk_walkers : ((uint64_t)1 << 31) / (4ull * o->max_length);
+  // Paths of one chunk take chunk * max_length * 4 bytes; default about 8 GiB
+  // (a small share of the 288 GB of HBM; fewer, larger launches).
+  uint64_t chunk = o->chunk_walkers ? o->chunk_walkers : ((uint64_t)1 << 33) / (4ull * o->max_length);
   if (chunk > total) chunk = total;
   if (chunk == 0) chunk = 1;
   if (chunk > (1ull << 30)) return host_fail(DGER_E_UNSUPPORTED, "chunk_walkers must be <= 2^30");
-  Dev d_row, d_col, d_cnb, d_path, d_len, d_off, d_corp, d_vcn, d_cnt, d_tmp;
+  Dev d_row, d_col, d_cnb, d_path, d_len, d_off, d_cnt, d_tmp, d_vcn;
   WALK_HIP(d_row.alloc((v_num + 1) * sizeof(int64_t)));
   WALK_HIP(d_col.alloc(nnz * sizeof(int32_t)));
   WALK_HIP(d_cnb.alloc(nnz * sizeof(int32_t)));
   WALK_HIP(d_path.alloc(chunk * o->max_length * sizeof(int32_t)));
   WALK_HIP(d_len.alloc(chunk * sizeof(u64)));
   WALK_HIP(d_off.alloc(chunk * sizeof(u64)));
-  WALK_HIP(d_corp.alloc(chunk * (o->max_length + 1ull) * sizeof(int32_t)));
   WALK_HIP(d_vcn.alloc(v_num * sizeof(int32_t)));
-  WALK_HIP(d_cnt.alloc(4 * sizeof(u64)));
+  WALK_HIP(d_cnt.alloc(5 * sizeof(u64)));
+  // The corpus grows in HBM: capacity starts at room for walks of
+  // 2 * min_length + 4 vertices per walker and doubles (device-to-device
+  // copy) when a chunk does not fit.
+  Dev d_corp;
+  const uint64_t guess = std::min<uint64_t>(o->max_length, 2ull * o->min_length + 4);
+  uint64_t corp_cap = std::min<uint64_t>(total, (uint64_t)1 << 31) * (guess + 1);
+  if (corp_cap == 0) corp_cap = 1;
+  WALK_HIP(d_corp.alloc(corp_cap * sizeof(int32_t)));
   size_t tmp_bytes = 0;
   WALK_HIP(hipMemcpy(d_row.p, rowptr, (v_num + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
   if (nnz) {
@@ -337,53 +357,68 @@ int dger_walk_run(const int64_t* rowptr, const int32_t* col, const int32_t* cnb,
     WALK_HIP(hipMemcpy(d_cnb.p, cnb, nnz * sizeof(int32_t), hipMemcpyHostToDevice));
   }
   WALK_HIP(hipMemset(d_vcn.p, 0, v_num * sizeof(int32_t)));
-  WALK_HIP(hipMemset(d_cnt.p, 0, 4 * sizeof(u64)));
+  WALK_HIP(hipMemset(d_cnt.p, 0, 5 * sizeof(u64)));
   hipEvent_t ev0, ev1;
   WALK_HIP(hipEventCreate(&ev0));
   WALK_HIP(hipEventCreate(&ev1));
   float walk_ms = 0.f;
   uint64_t out_len = 0;
-  bool overflow = false;
-  for (uint64_t done = 0; done < total; done += chunk) {
+  int rc = DGER_OK;
+  for (uint64_t done = 0; done < total && rc == DGER_OK; done += chunk) {
     const uint64_t n = total - done < chunk ? total - done : chunk;
-    WALK_HIP(hipEventRecord(ev0, 0));
-    hipLaunchKernelGGL(k_walk, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, d_row.as<int64_t>(),
-                       d_col.as<int32_t>(), d_cnb.as<int32_t>(), v_num, (u64)(o->first_walker + done), (u64)n,
-                       o->min_length, o->max_length, o->r_threshold, (u64)o->seed, d_path.as<int32_t>(),
-                       d_len.as<u64>(), d_cnt.as<u64>());
-    WALK_HIP(hipGetLastError());
-    WALK_HIP(hipEventRecord(ev1, 0));
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipError_t e = hipEventRecord(ev0, 0);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_walk, dim3(blocks), dim3(256), 0, 0, d_row.as<int64_t>(), d_col.as<int32_t>(),
+                         d_cnb.as<int32_t>(), v_num, (u64)(o->first_walker + done), (u64)n, o->min_length,
+                         o->max_length, o->r_threshold, (u64)o->seed, d_path.as<int32_t>(), d_len.as<u64>(),
+                         d_cnt.as<u64>());
+      e = hipGetLastError();
+    }
     // inclusive sum of len + 1 (one separator per walk)
     size_t need = 0;
-    WALK_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, need, d_len.as<u64>(), d_off.as<u64>(), (int)n, 0));
-    if (need > tmp_bytes) {
+    if (e == hipSuccess) e = hipcub::DeviceScan::InclusiveSum(nullptr, need, d_len.as<u64>(), d_off.as<u64>(), (int)n, 0);
+    if (e == hipSuccess && need > tmp_bytes) {
       if (d_tmp.p) (void)hipFree(d_tmp.p);
       d_tmp.p = nullptr;
-      WALK_HIP(d_tmp.alloc(need));
+      e = d_tmp.alloc(need);
       tmp_bytes = need;
     }
-    WALK_HIP(hipcub::DeviceScan::InclusiveSum(d_tmp.p, tmp_bytes, d_len.as<u64>(), d_off.as<u64>(), (int)n, 0));
-    hipLaunchKernelGGL(k_walk_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, d_path.as<int32_t>(),
-                       d_len.as<u64>(), d_off.as<u64>(), (u64)n, d_corp.as<int32_t>(), d_vcn.as<int32_t>());
-    WALK_HIP(hipGetLastError());
+    if (e == hipSuccess)
+      e = hipcub::DeviceScan::InclusiveSum(d_tmp.p, tmp_bytes, d_len.as<u64>(), d_off.as<u64>(), (int)n, 0);
     u64 chunk_len = 0;
-    WALK_HIP(hipMemcpy(&chunk_len, d_off.as<u64>() + (n - 1), sizeof(u64), hipMemcpyDeviceToHost));
-    float ms = 0.f;
-    WALK_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-    walk_ms += ms;
-    if (!overflow && out_len + chunk_len <= corpus_cap) {
-      WALK_HIP(hipMemcpy(corpus + out_len, d_corp.p, chunk_len * sizeof(int32_t), hipMemcpyDeviceToHost));
-    } else {
-      overflow = true;
+    if (e == hipSuccess) e = hipMemcpy(&chunk_len, d_off.as<u64>() + (n - 1), sizeof(u64), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && out_len + chunk_len > corp_cap) {
+      uint64_t cap2 = corp_cap * 2;
+      while (cap2 < out_len + chunk_len) cap2 *= 2;
+      Dev bigger;
+      e = bigger.alloc(cap2 * sizeof(int32_t));
+      if (e == hipSuccess) e = hipMemcpy(bigger.p, d_corp.p, out_len * sizeof(int32_t), hipMemcpyDeviceToDevice);
+      if (e == hipSuccess) {
+        std::swap(bigger.p, d_corp.p);
+        corp_cap = cap2;
+      }
     }
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_walk_pack, dim3(blocks), dim3(256), 0, 0, d_path.as<int32_t>(), d_len.as<u64>(),
+                         d_off.as<u64>(), (u64)n, d_corp.as<int32_t>() + out_len, d_vcn.as<int32_t>(),
+                         d_cnt.as<u64>() + 4);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev1, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+    if (e != hipSuccess) rc = host_fail(DGER_E_HIP, std::string("walk chunk: ") + hipGetErrorString(e));
+    walk_ms += ms;
     out_len += chunk_len;
   }
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
+  if (rc != DGER_OK) return rc;
   WALK_HIP(hipMemcpy(vertex_cn, d_vcn.p, v_num * sizeof(int32_t), hipMemcpyDeviceToHost));
-  u64 cnt[4] = {0, 0, 0, 0};
+  u64 cnt[5] = {0, 0, 0, 0, 0};
   WALK_HIP(hipMemcpy(cnt, d_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost));
-  *corpus_len = out_len;
   if (st) {
     st->walks = total;
     st->tokens = cnt[0];
@@ -392,8 +427,54 @@ int dger_walk_run(const int64_t* rowptr, const int32_t* col, const int32_t* cnb,
     st->capped = cnt[3];
     st->walk_ms = walk_ms;
   }
-  if (overflow) return host_fail(DGER_E_ARG, "corpus_cap too small; *corpus_len holds the length needed");
+  dger_walk_corpus* h = new (std::nothrow) dger_walk_corpus;
+  if (!h) return host_fail(DGER_E_ALLOC, "corpus handle allocation failed");
+  h->d = d_corp.as<int32_t>();
+  d_corp.p = nullptr;
+  h->n_tokens = out_len;
+  h->max_walk = (uint32_t)cnt[4];
+  h->v_num = v_num;
+  h->device = o->device;
+  *out = h;
   return DGER_OK;
+}
+
+int dger_walk_corpus_info(const dger_walk_corpus* h, uint64_t* n_tokens, uint32_t* max_walk) {
+  if (!h) return host_fail(DGER_E_ARG, "null corpus handle");
+  if (n_tokens) *n_tokens = h->n_tokens;
+  if (max_walk) *max_walk = h->max_walk;
+  return DGER_OK;
+}
+
+int dger_walk_corpus_download(const dger_walk_corpus* h, int32_t* corpus, uint64_t corpus_cap) {
+  if (!h) return host_fail(DGER_E_ARG, "null corpus handle");
+  if (corpus_cap < h->n_tokens) return host_fail(DGER_E_ARG, "corpus_cap below the corpus length");
+  if (h->n_tokens && !corpus) return host_fail(DGER_E_ARG, "null corpus");
+  WALK_HIP(hipSetDevice(h->device));
+  if (h->n_tokens) WALK_HIP(hipMemcpy(corpus, h->d, h->n_tokens * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return DGER_OK;
+}
+
+void dger_walk_corpus_destroy(dger_walk_corpus* h) {
+  if (!h) return;
+  if (h->d && hipSetDevice(h->device) == hipSuccess) (void)hipFree(h->d);
+  delete h;
+}
+
+int dger_walk_run(const int64_t* rowptr, const int32_t* col, const int32_t* cnb, uint32_t v_num,
+                  const dger_walk_opts* o, int32_t* corpus, uint64_t corpus_cap, uint64_t* corpus_len,
+                  int32_t* vertex_cn, dger_walk_stats* st) {
+  if (!corpus_len) return host_fail(DGER_E_ARG, "null argument");
+  dger_walk_corpus* h = nullptr;
+  if (int rc = dger_walk_run_device(rowptr, col, cnb, v_num, o, &h, vertex_cn, st)) return rc;
+  *corpus_len = h->n_tokens;
+  int rc = DGER_OK;
+  if (corpus_cap < h->n_tokens)
+    rc = host_fail(DGER_E_ARG, "corpus_cap too small; *corpus_len holds the length needed");
+  else
+    rc = dger_walk_corpus_download(h, corpus, corpus_cap);
+  dger_walk_corpus_destroy(h);
+  return rc;
 }
 
 }  // extern "C"

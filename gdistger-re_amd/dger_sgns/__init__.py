@@ -126,6 +126,12 @@ _SIGS = {
     "dger_walk_common_neighbors": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _i32]),
     "dger_walk_run": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.POINTER(WalkOpts), _vp, C.c_uint64,
                                 C.POINTER(C.c_uint64), _vp, C.POINTER(WalkStats)]),
+    "dger_walk_run_device": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.POINTER(WalkOpts), C.POINTER(_vp), _vp,
+                                       C.POINTER(WalkStats)]),
+    "dger_walk_corpus_info": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "dger_walk_corpus_download": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "dger_walk_corpus_destroy": (None, [_vp]),
+    "dger_ctx_set_corpus_walks": (C.c_int, [_vp, _vp]),
 }
 EXPORTS = tuple(_SIGS)
 
@@ -214,6 +220,54 @@ def common_neighbors(rowptr, col, device: int = 0) -> np.ndarray:
     return out
 
 
+class WalkCorpus:
+    """Walks left in HBM by dger_walk_run_device."""
+
+    def __init__(self, h):
+        self._h = h
+
+    def info(self):
+        n, m = C.c_uint64(0), C.c_uint32(0)
+        _check(lib().dger_walk_corpus_info(self._h, C.byref(n), C.byref(m)))
+        return int(n.value), int(m.value)
+
+    def download(self) -> np.ndarray:
+        n, _ = self.info()
+        out = np.empty(max(n, 1), np.int32)
+        _check(lib().dger_walk_corpus_download(self._h, _ptr(out), n))
+        return out[:n]
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().dger_walk_corpus_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def walk_device(rowptr, col, cnb, opts: WalkOpts):
+    """Like walk(), but the corpus stays in HBM: returns (WalkCorpus, vertex_cn, stats)."""
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    col = _i32arr(col)
+    cnb = _i32arr(cnb)
+    vcn = np.zeros(rowptr.size - 1, np.int32)
+    h = _vp()
+    st = WalkStats()
+    _check(lib().dger_walk_run_device(_ptr(rowptr), _ptr(col), _ptr(cnb), rowptr.size - 1, C.byref(opts),
+                                      C.byref(h), _ptr(vcn), C.byref(st)))
+    return WalkCorpus(h), vcn, st
+
+
 def walk(rowptr, col, cnb, opts: WalkOpts):
     """Information-oriented walks on the GPU (dger_walk_run).  Returns the
     corpus (vertex ids, -1 after every walk), vertex_cn and the stats."""
@@ -265,6 +319,10 @@ class Trainer:
     def set_corpus(self, corpus) -> None:
         c = _i32arr(corpus)
         _check(lib().dger_ctx_set_corpus(self._h, _ptr(c), c.size))
+
+    def set_corpus_walks(self, walks: "WalkCorpus") -> None:
+        """Corpus from walks left in HBM (dger_ctx_set_corpus_walks)."""
+        _check(lib().dger_ctx_set_corpus_walks(self._h, walks._h))
 
     def reset_model(self) -> None:
         _check(lib().dger_ctx_reset_model(self._h))

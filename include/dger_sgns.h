@@ -136,6 +136,14 @@ int dger_ctx_create(const dger_sgns_opts *opts, const int32_t *vertex_cn, uint32
                     dger_ctx **out);
 /* Copies the corpus to device memory (validated on the host first). */
 int dger_ctx_set_corpus(dger_ctx *ctx, const int32_t *corpus, uint64_t n_tokens);
+
+/* Hands walks left in HBM by dger_walk_run_device (include/dger_walk.h) to
+ * the trainer without a host round trip (a device-to-device copy; the
+ * handle stays the caller's).  Replaces the reference's text dump and
+ * reload of local_corpus (include/path.hpp:67-119).  The walks must be on
+ * the context's device and use ids below its v_num. */
+struct dger_walk_corpus;
+int dger_ctx_set_corpus_walks(dger_ctx *ctx, const struct dger_walk_corpus *walks);
 /* Re-initialises Wih (LCG, seed 1) and zeroes Woh. */
 int dger_ctx_reset_model(dger_ctx *ctx);
 /* One training run (all -iter epochs) with `workers` workers (0 = opts).

@@ -40,7 +40,7 @@ typedef struct dger_walk_opts {
   double r_threshold;     /* a walk ends once R^2 < r_threshold or R < 0 (R = 0.999) */
   uint64_t seed;          /* random stream: splitmix64 keyed by (seed, walker id, draw index) */
   int32_t device;         /* HIP device ordinal */
-  uint32_t chunk_walkers; /* walkers per launch; 0 = as many as fit 2 GiB of paths */
+  uint32_t chunk_walkers; /* walkers per launch; 0 = as many as fit 8 GiB of paths */
 } dger_walk_opts;
 
 typedef struct dger_walk_stats {
@@ -49,8 +49,11 @@ typedef struct dger_walk_stats {
   uint64_t moves;   /* neighbour draws that reached the acceptance test */
   uint64_t traces;  /* moves that stepped to the neighbour and back */
   uint64_t capped;  /* walks cut at max_length */
-  double walk_ms;   /* time of the walk kernels */
+  double walk_ms;   /* device time of the walk, scan and corpus-packing kernels */
 } dger_walk_stats;
+
+/* Walks kept in device memory (HBM) for the trainer; see dger_walk_run_device. */
+typedef struct dger_walk_corpus dger_walk_corpus;
 
 void dger_walk_default_opts(dger_walk_opts* o);
 
@@ -67,6 +70,22 @@ int dger_walk_common_neighbors(const int64_t* rowptr, const int32_t* col, uint32
 int dger_walk_run(const int64_t* rowptr, const int32_t* col, const int32_t* cnb, uint32_t v_num,
                   const dger_walk_opts* o, int32_t* corpus, uint64_t corpus_cap, uint64_t* corpus_len,
                   int32_t* vertex_cn, dger_walk_stats* st);
+
+/* Same walks, left in HBM: *out receives a handle to the corpus on o->device
+ * (the reference's local_corpus, path.hpp:96,110, without the text dump);
+ * vertex_cn[v_num] is written to the host.  Hand it to the trainer with
+ * dger_ctx_set_corpus_walks (include/dger_sgns.h), which copies it on the
+ * device; free it with dger_walk_corpus_destroy. */
+int dger_walk_run_device(const int64_t* rowptr, const int32_t* col, const int32_t* cnb, uint32_t v_num,
+                         const dger_walk_opts* o, dger_walk_corpus** out, int32_t* vertex_cn, dger_walk_stats* st);
+
+/* Token count (separators included) and the longest walk of a handle. */
+int dger_walk_corpus_info(const dger_walk_corpus* h, uint64_t* n_tokens, uint32_t* max_walk);
+
+/* Copies the corpus to the host (corpus_cap >= its token count). */
+int dger_walk_corpus_download(const dger_walk_corpus* h, int32_t* corpus, uint64_t corpus_cap);
+
+void dger_walk_corpus_destroy(dger_walk_corpus* h);
 
 #ifdef __cplusplus
 }

@@ -43,19 +43,31 @@ def corr_trace(path) -> tuple[np.ndarray, np.ndarray]:
     return H, R
 
 
-def walk(rowptr, col, cnb, n_walkers, first=0, min_length=20, max_length=1024, r_threshold=0.999, seed=1):
-    """Returns (corpus int32 with -1 after each walk, vertex_cn, stats dict)."""
+def walk(rowptr, col, cnb, n_walkers, first=0, min_length=20, max_length=1024, r_threshold=0.999, seed=1,
+         chunk=100_000):
+    """Returns (corpus int32 with -1 after each walk, vertex_cn, stats dict);
+    walkers run in chunks so the host buffer stays chunk * (max_length + 1)."""
     rowptr = np.ascontiguousarray(rowptr, np.int64)
     col = np.ascontiguousarray(col, np.int32)
     cnb = np.ascontiguousarray(cnb, np.int32)
     V = rowptr.size - 1
-    cap = int(n_walkers) * (max_length + 1)
-    corpus = np.empty(max(cap, 1), np.int32)
-    vcn = np.zeros(V, np.int32)
-    st = np.zeros(5, np.uint64)
-    n = _load().orc_walk_run(rowptr.ctypes.data, col.ctypes.data, cnb.ctypes.data, V, first, n_walkers,
-                             min_length, max_length, r_threshold, seed, corpus.ctypes.data, cap,
-                             vcn.ctypes.data, st.ctypes.data)
-    assert n >= 0
     keys = ("walks", "tokens", "moves", "traces", "capped")
-    return corpus[:n].copy(), vcn, {k: int(v) for k, v in zip(keys, st)}
+    total = {k: 0 for k in keys}
+    vcn = np.zeros(V, np.int64)
+    parts = []
+    part_vcn = np.zeros(V, np.int32)
+    st = np.zeros(5, np.uint64)
+    for lo in range(0, int(n_walkers), chunk):
+        n = min(chunk, int(n_walkers) - lo)
+        cap = n * (max_length + 1)
+        corpus = np.empty(max(cap, 1), np.int32)
+        got = _load().orc_walk_run(rowptr.ctypes.data, col.ctypes.data, cnb.ctypes.data, V, first + lo, n,
+                                   min_length, max_length, r_threshold, seed, corpus.ctypes.data, cap,
+                                   part_vcn.ctypes.data, st.ctypes.data)
+        assert got >= 0
+        parts.append(corpus[:got].copy())
+        vcn += part_vcn
+        for k, x in zip(keys, st):
+            total[k] += int(x)
+    corpus = np.concatenate(parts) if parts else np.empty(0, np.int32)
+    return corpus, vcn.astype(np.int32), total
