@@ -204,11 +204,25 @@ __global__ __launch_bounds__(256) void k_walk_pack(const int32_t* __restrict__ p
 
 // |N(u) ∩ N(v)| for every directed edge (u, v = col[e]); rows sorted
 // ascending without repeats.  Each element of the shorter row is searched
-// in the longer one.
+// in the longer one.  An edge whose shorter row has more than kLaneRow
+// entries (hub to hub) is appended to `heavy` and left to
+// k_common_neighbors_heavy, so no lane carries a long serial search.
+constexpr int64_t kLaneRow = 32;
+
+__device__ __forceinline__ int32_t row_has(const int32_t* __restrict__ col, int64_t lo, int64_t hi, int32_t x) {
+  const int64_t end = hi;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (col[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < end && col[lo] == x;
+}
+
 __global__ __launch_bounds__(256) void k_common_neighbors(const int64_t* __restrict__ rowptr,
                                                           const int32_t* __restrict__ col,
                                                           const int32_t* __restrict__ src, u64 nnz,
-                                                          int32_t* __restrict__ cnb) {
+                                                          int32_t* __restrict__ cnb, u64* __restrict__ heavy,
+                                                          u64* __restrict__ n_heavy) {
   const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nnz) return;
   const int32_t u = src[e], v = col[e];
@@ -217,17 +231,36 @@ __global__ __launch_bounds__(256) void k_common_neighbors(const int64_t* __restr
     int64_t x = a0; a0 = b0; b0 = x;
     x = a1; a1 = b1; b1 = x;
   }
-  int32_t k = 0;
-  for (int64_t i = a0; i < a1; ++i) {
-    const int32_t x = col[i];
-    int64_t lo = b0, hi = b1;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (col[mid] < x) lo = mid + 1; else hi = mid;
-    }
-    k += lo < b1 && col[lo] == x;
+  if (a1 - a0 > kLaneRow) {
+    heavy[atomicAdd(n_heavy, 1ull)] = e;
+    return;
   }
+  int32_t k = 0;
+  for (int64_t i = a0; i < a1; ++i) k += row_has(col, b0, b1, col[i]);
   cnb[e] = k;
+}
+
+// One wavefront per heavy edge: lanes split the shorter row.
+__global__ __launch_bounds__(256) void k_common_neighbors_heavy(const int64_t* __restrict__ rowptr,
+                                                                const int32_t* __restrict__ col,
+                                                                const int32_t* __restrict__ src,
+                                                                const u64* __restrict__ heavy, u64 n_heavy,
+                                                                int32_t* __restrict__ cnb) {
+  const u64 h = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (h >= n_heavy) return;   // whole wavefronts leave together
+  const u64 e = heavy[h];
+  const int32_t u = src[e], v = col[e];
+  int64_t a0 = rowptr[u], a1 = rowptr[u + 1], b0 = rowptr[v], b1 = rowptr[v + 1];
+  if (a1 - a0 > b1 - b0) {
+    int64_t x = a0; a0 = b0; b0 = x;
+    x = a1; a1 = b1; b1 = x;
+  }
+  int32_t k = 0;
+  for (int64_t i = a0 + lane; i < a1; i += 64) k += row_has(col, b0, b1, col[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) k += __shfl_xor(k, off, 64);
+  if (lane == 0) cnb[e] = k;
 }
 
 __global__ void k_row_ids(const int64_t* __restrict__ rowptr, uint32_t V, int32_t* __restrict__ src) {
@@ -295,19 +328,32 @@ int dger_walk_common_neighbors(const int64_t* rowptr, const int32_t* col, uint32
   if (nnz && !cnb_out) return host_fail(DGER_E_ARG, "null cnb_out");
   if (nnz == 0) return DGER_OK;
   WALK_HIP(hipSetDevice(device));
-  Dev d_row, d_col, d_src, d_cnb;
+  Dev d_row, d_col, d_src, d_cnb, d_heavy, d_nh;
   WALK_HIP(d_row.alloc((v_num + 1) * sizeof(int64_t)));
   WALK_HIP(d_col.alloc(nnz * sizeof(int32_t)));
   WALK_HIP(d_src.alloc(nnz * sizeof(int32_t)));
   WALK_HIP(d_cnb.alloc(nnz * sizeof(int32_t)));
+  WALK_HIP(d_heavy.alloc(nnz * sizeof(u64)));
+  WALK_HIP(d_nh.alloc(sizeof(u64)));
   WALK_HIP(hipMemcpy(d_row.p, rowptr, (v_num + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
   WALK_HIP(hipMemcpy(d_col.p, col, nnz * sizeof(int32_t), hipMemcpyHostToDevice));
+  WALK_HIP(hipMemset(d_nh.p, 0, sizeof(u64)));
   hipLaunchKernelGGL(k_row_ids, dim3((v_num + 255) / 256), dim3(256), 0, 0, d_row.as<int64_t>(), v_num,
                      d_src.as<int32_t>());
   WALK_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_common_neighbors, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, 0,
-                     d_row.as<int64_t>(), d_col.as<int32_t>(), d_src.as<int32_t>(), (u64)nnz, d_cnb.as<int32_t>());
+                     d_row.as<int64_t>(), d_col.as<int32_t>(), d_src.as<int32_t>(), (u64)nnz, d_cnb.as<int32_t>(),
+                     d_heavy.as<u64>(), d_nh.as<u64>());
   WALK_HIP(hipGetLastError());
+  u64 n_heavy = 0;
+  WALK_HIP(hipMemcpy(&n_heavy, d_nh.p, sizeof(u64), hipMemcpyDeviceToHost));
+  if (n_heavy) {
+    // four wavefronts (heavy edges) per 256-lane workgroup
+    hipLaunchKernelGGL(k_common_neighbors_heavy, dim3((unsigned)((n_heavy + 3) / 4)), dim3(256), 0, 0,
+                       d_row.as<int64_t>(), d_col.as<int32_t>(), d_src.as<int32_t>(), d_heavy.as<u64>(), n_heavy,
+                       d_cnb.as<int32_t>());
+    WALK_HIP(hipGetLastError());
+  }
   WALK_HIP(hipMemcpy(cnb_out, d_cnb.p, nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
   return DGER_OK;
 }

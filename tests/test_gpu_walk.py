@@ -56,6 +56,15 @@ def test_common_neighbors_rmat():
     print(f"common neighbours V={n} nnz={col.size}", flush=True)
     got = G.common_neighbors(rowptr, col)
     assert np.array_equal(got, common_neighbors_np(rowptr, col))
+    # every row longer than one lane's share: all edges take the wavefront path
+    iu = np.triu_indices(80, 1)
+    rowptr, col = sorted_csr(80, iu[0].astype(np.int32), iu[1].astype(np.int32))
+    assert np.all(G.common_neighbors(rowptr, col) == 78)
+    # a hub joined to hubs and to leaves: both paths in one call
+    u = np.concatenate([np.zeros(40), np.ones(40), np.arange(2, 42), [0, 0]]).astype(np.int32)
+    v = np.concatenate([np.arange(2, 42), np.arange(2, 42), np.full(40, 100), [1, 100]]).astype(np.int32)
+    rowptr, col = sorted_csr(200, u, v)
+    assert np.array_equal(G.common_neighbors(rowptr, col), common_neighbors_np(rowptr, col))
 
 
 def test_walk_matches_oracle_er():
