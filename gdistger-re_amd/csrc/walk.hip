@@ -96,12 +96,15 @@ struct Corr {
 
 // One thread per walker.  path is step-major over the chunk's n walkers;
 // len1[t] = walk length + 1 (the walk's share of the corpus, separator
-// included).
+// included).  vertex_cn gets one count per vertex written: visits of the
+// current vertex (the start and every trace-return) gather in a register
+// and are added when the walker moves on or stops, other visits are added
+// at once; the adds return nothing, so they overlap the walk's loads.
 __global__ __launch_bounds__(256) void k_walk(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                               const int32_t* __restrict__ cnb, uint32_t V, u64 first, u64 n,
                                               uint32_t min_length, uint32_t max_length, double r_thr, u64 seed,
                                               int32_t* __restrict__ path, u64* __restrict__ len1,
-                                              u64* __restrict__ counters) {
+                                              int32_t* __restrict__ vertex_cn, u64* __restrict__ counters) {
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   u64 tokens = 0, moves = 0, traces = 0, capped = 0;
   if (t < n) {
@@ -111,6 +114,7 @@ __global__ __launch_bounds__(256) void k_walk(const int64_t* __restrict__ rowptr
     uint32_t len = 1, step = 0;
     u64 ctr = 0;
     int self_draws = 0;
+    int32_t cur_visits = 1;   // visits of cur not yet added to vertex_cn
     Corr c{0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 1.0, 1.0, 1.0};
     auto seen = [&](uint32_t v) {
       uint32_t k = 0;
@@ -142,6 +146,8 @@ __global__ __launch_bounds__(256) void k_walk(const int64_t* __restrict__ rowptr
       if (len >= max_length) { capped = 1; break; }
       const double f1 = seen(cand);
       path[(u64)len * n + t] = (int32_t)cand;
+      if (cand == cur) ++cur_visits;
+      else atomicAdd(&vertex_cn[cand], 1);
       ++len; ++step;
       c.iterate(f1);
       if (ends()) break;
@@ -149,13 +155,17 @@ __global__ __launch_bounds__(256) void k_walk(const int64_t* __restrict__ rowptr
         if (len >= max_length) { capped = 1; break; }
         const double f2 = seen(cur);
         path[(u64)len * n + t] = (int32_t)cur;
+        ++cur_visits;
         ++len; ++step;
         c.iterate(f2);
         if (ends()) break;
-      } else {
+      } else if (cand != cur) {
+        atomicAdd(&vertex_cn[cur], cur_visits);   // cand's visit is already counted
         cur = cand;
+        cur_visits = 0;
       }
     }
+    if (cur_visits) atomicAdd(&vertex_cn[cur], cur_visits);
     len1[t] = (u64)len + 1;
     tokens = len;
   }
@@ -180,17 +190,13 @@ __global__ __launch_bounds__(256) void k_walk(const int64_t* __restrict__ rowptr
 // longest walk (one atomic per wavefront).
 __global__ __launch_bounds__(256) void k_walk_pack(const int32_t* __restrict__ path, const u64* __restrict__ len1,
                                                    const u64* __restrict__ off_incl, u64 n, int32_t* __restrict__ corpus,
-                                                   int32_t* __restrict__ vertex_cn, u64* __restrict__ max_walk) {
+                                                   u64* __restrict__ max_walk) {
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   u64 longest = 0;
   if (t < n) {
     const u64 o = t == 0 ? 0 : off_incl[t - 1];
     const uint32_t len = (uint32_t)(len1[t] - 1);
-    for (uint32_t i = 0; i < len; ++i) {
-      const int32_t v = path[(u64)i * n + t];
-      corpus[o + i] = v;
-      atomicAdd(&vertex_cn[v], 1);
-    }
+    for (uint32_t i = 0; i < len; ++i) corpus[o + i] = path[(u64)i * n + t];
     corpus[o + len] = -1;
     longest = len;
   }
@@ -418,7 +424,7 @@ int dger_walk_run_device(const int64_t* rowptr, const int32_t* col, const int32_
       hipLaunchKernelGGL(k_walk, dim3(blocks), dim3(256), 0, 0, d_row.as<int64_t>(), d_col.as<int32_t>(),
                          d_cnb.as<int32_t>(), v_num, (u64)(o->first_walker + done), (u64)n, o->min_length,
                          o->max_length, o->r_threshold, (u64)o->seed, d_path.as<int32_t>(), d_len.as<u64>(),
-                         d_cnt.as<u64>());
+                         d_vcn.as<int32_t>(), d_cnt.as<u64>());
       e = hipGetLastError();
     }
     // inclusive sum of len + 1 (one separator per walk)
@@ -447,8 +453,7 @@ int dger_walk_run_device(const int64_t* rowptr, const int32_t* col, const int32_
     }
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_walk_pack, dim3(blocks), dim3(256), 0, 0, d_path.as<int32_t>(), d_len.as<u64>(),
-                         d_off.as<u64>(), (u64)n, d_corp.as<int32_t>() + out_len, d_vcn.as<int32_t>(),
-                         d_cnt.as<u64>() + 4);
+                         d_off.as<u64>(), (u64)n, d_corp.as<int32_t>() + out_len, d_cnt.as<u64>() + 4);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ev1, 0);

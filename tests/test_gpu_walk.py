@@ -74,6 +74,9 @@ def test_walk_matches_oracle_er():
     cnb = G.common_neighbors(rowptr, col)
     check_same(rowptr, col, cnb, n_walkers=5 * n, seed=3)
     check_same(rowptr, col, cnb, n_walkers=3000, first_walker=777, min_length=5, seed=9)
+    # walks longer than the packing tile (per-lane copy) next to short ones
+    check_same(rowptr, col, cnb, n_walkers=3000, min_length=60, seed=2)
+    check_same(rowptr, col, cnb, n_walkers=1000, min_length=37, seed=2)
 
 
 def test_walk_matches_oracle_rmat_chunks():
