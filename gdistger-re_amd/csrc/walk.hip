@@ -94,6 +94,8 @@ struct Corr {
   }
 };
 
+constexpr uint32_t kLdsSteps = 32;   // 32 KB of LDS per 256-lane workgroup
+
 // One thread per walker.  path is step-major over the chunk's n walkers;
 // len1[t] = walk length + 1 (the walk's share of the corpus, separator
 // included).  vertex_cn gets one count per vertex written: visits of the
@@ -105,21 +107,32 @@ __global__ __launch_bounds__(256) void k_walk(const int64_t* __restrict__ rowptr
                                               uint32_t min_length, uint32_t max_length, double r_thr, u64 seed,
                                               int32_t* __restrict__ path, u64* __restrict__ len1,
                                               int32_t* __restrict__ vertex_cn, u64* __restrict__ counters) {
+  // each lane's column holds its walk's first kLdsSteps vertices
+  __shared__ int32_t lpath[kLdsSteps][256];
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   u64 tokens = 0, moves = 0, traces = 0, capped = 0;
   if (t < n) {
     const u64 w = first + t;
     uint32_t cur = (uint32_t)(w % V);
     path[t] = (int32_t)cur;
+    lpath[0][threadIdx.x] = (int32_t)cur;
     uint32_t len = 1, step = 0;
     u64 ctr = 0;
     int self_draws = 0;
     int32_t cur_visits = 1;   // visits of cur not yet added to vertex_cn
     Corr c{0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 1.0, 1.0, 1.0};
+    // earlier visits of v: the first kLdsSteps steps from the LDS mirror,
+    // the rest (long walks) from the path in HBM
     auto seen = [&](uint32_t v) {
       uint32_t k = 0;
-      for (uint32_t i = 0; i < len; ++i) k += path[(u64)i * n + t] == (int32_t)v;
+      const uint32_t m = len < kLdsSteps ? len : kLdsSteps;
+      for (uint32_t i = 0; i < m; ++i) k += lpath[i][threadIdx.x] == (int32_t)v;
+      for (uint32_t i = kLdsSteps; i < len; ++i) k += path[(u64)i * n + t] == (int32_t)v;
       return (double)k;
+    };
+    auto put = [&](uint32_t x) {
+      path[(u64)len * n + t] = (int32_t)x;
+      if (len < kLdsSteps) lpath[len][threadIdx.x] = (int32_t)x;
     };
     auto ends = [&]() { return step > min_length && (c.R * c.R < r_thr || c.R < 0.0); };
     while (true) {
@@ -145,7 +158,7 @@ __global__ __launch_bounds__(256) void k_walk(const int64_t* __restrict__ rowptr
       traces += trace;
       if (len >= max_length) { capped = 1; break; }
       const double f1 = seen(cand);
-      path[(u64)len * n + t] = (int32_t)cand;
+      put(cand);
       if (cand == cur) ++cur_visits;
       else atomicAdd(&vertex_cn[cand], 1);
       ++len; ++step;
@@ -154,7 +167,7 @@ __global__ __launch_bounds__(256) void k_walk(const int64_t* __restrict__ rowptr
       if (trace) {
         if (len >= max_length) { capped = 1; break; }
         const double f2 = seen(cur);
-        path[(u64)len * n + t] = (int32_t)cur;
+        put(cur);
         ++cur_visits;
         ++len; ++step;
         c.iterate(f2);
