@@ -154,3 +154,33 @@ int main() {
                     "-ldger_sgns", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_walk_header_is_plain_c(tmp_path):
+    """A C99 caller of dger_walk.h and dger_sgns.h compiles, links, reads the
+    walk defaults (minLength 20, R 0.999: include/walk.hpp:404, :24) and gets
+    DGER_E_ARG for a graph with an out-of-range neighbour, without a device."""
+    src = tmp_path / "walk_caller.c"
+    src.write_text(r'''
+#include "dger_sgns.h"
+#include "dger_walk.h"
+#include <stdio.h>
+int main(void) {
+  dger_walk_opts o;
+  dger_walk_default_opts(&o);
+  if (o.min_length != 20 || o.r_threshold != 0.999 || o.max_length != 1024) return 2;
+  int64_t rowptr[3] = {0, 1, 2};
+  int32_t col[2] = {1, 7}, cnb[2] = {0, 0}, vcn[2], corpus[8];
+  uint64_t n = 0;
+  o.n_walkers = 2;
+  int rc = dger_walk_run(rowptr, col, cnb, 2, &o, corpus, 8, &n, vcn, NULL);
+  printf("%d %s\n", rc, dger_last_error());
+  return rc == DGER_E_ARG ? 0 : 1;
+}
+''')
+    libdir = os.path.dirname(G.LIB_PATH)
+    exe = tmp_path / "walk_caller"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", str(src), "-I", os.path.join(ROOT, "include"),
+                    "-L", libdir, "-ldger_sgns", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
