@@ -34,18 +34,19 @@
 //
 // Layout in HBM: per chunk of walkers the paths are stored step-major,
 // path[i * n + t] for walker t, so lanes of a wavefront that are at the same
-// step write and scan neighbouring words.  The earlier-visit count of a
-// vertex is a scan of the walker's own path (walks are tens of vertices).
+// step write neighbouring words.  The earlier-visit count of a vertex is a
+// scan of the walker's own path (walks are tens of vertices), the first 32
+// steps from a per-lane LDS mirror.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <algorithm>
 #include <cstring>
 #include <new>
-#include <utility>
 #include <string>
+#include <utility>
 
 #include "dger_sgns.h"
 #include "dger_walk.h"
