@@ -715,6 +715,9 @@ int dger_ctx_set_model(dger_ctx* c, const float* wih, const float* woh) {
 static int link_scores_device(dger_ctx* c, const int32_t* pairs, uint64_t n) {
   if (!c) return fail(DGER_E_ARG, "null ctx");
   if (!pairs && n) return fail(DGER_E_ARG, "null pairs");
+  for (uint64_t i = 0; i < 2 * n; i++)
+    if (pairs[i] < 0 || (uint32_t)pairs[i] >= c->V)
+      return fail(DGER_E_ARG, "pair " + std::to_string(i / 2) + " holds a vertex id outside [0, v_num)");
   DGER_HIP(hipSetDevice(c->o.device));
   if (c->link_pairs_d.ensure(n * 8) || c->link_scores_d.ensure(n * 4))
     return fail(DGER_E_ALLOC, "link evaluation buffers");

@@ -350,30 +350,37 @@ __device__ inline void first_occurrence(const int* v, int* first, int n, int lan
   }
 }
 
-// first[p] as above through an LDS hash table tab of 2^cap_log2 words, each
-// (value << 11 | first position); the smallest position of a value wins by
-// atomic min.  Needs values < 2^21 and n <= 3/4 of the table (<= 2048).
+// first[p] as above through an LDS hash table of 2^cap_log2 slots: the
+// value of each slot in tab[0, cap), the smallest position holding it in
+// tab[cap, 2 cap) (atomic min).  Values are non-negative; n must stay below
+// 3/4 of the slot count so that probes end quickly.
 __device__ inline void first_occurrence_hash(const int* v, int* first, int n, int lane, unsigned* tab, int cap_log2) {
   const unsigned cap = 1u << cap_log2, empty = 0xFFFFFFFFu;
-  for (unsigned x = lane; x < cap; x += 64) tab[x] = empty;
+  unsigned* const keys = tab;
+  unsigned* const pos = tab + cap;
+  for (unsigned x = lane; x < cap; x += 64) {
+    keys[x] = empty;
+    pos[x] = empty;
+  }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   for (int c0 = 0; c0 < n; c0 += 64) {
     const int p = c0 + lane;
     if (p < n) {
       const unsigned key = (unsigned)v[p];
-      const unsigned pk = (key << 11) | (unsigned)p;
       unsigned h = (key * 0x9E3779B1u) >> (32 - cap_log2);
       for (;;) {
-        const unsigned old = atomicCAS(&tab[h], empty, pk);
-        if (old == empty) break;
-        if ((old >> 11) == key) { atomicMin(&tab[h], pk); break; }
+        const unsigned old = atomicCAS(&keys[h], empty, key);
+        if (old == empty || old == key) {
+          atomicMin(&pos[h], (unsigned)p);
+          break;
+        }
         h = (h + 1) & (cap - 1);
       }
       first[p] = (int)h;
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  for (int p = lane; p < n; p += 64) first[p] = (int)(tab[first[p]] & 0x7FFu);
+  for (int p = lane; p < n; p += 64) first[p] = (int)pos[first[p]];
 }
 
 // For each position p < n: first, previous and next position holding the
@@ -830,7 +837,7 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
   constexpr int B0 = std::is_same<R, float>::value ? 4 : 16;
   constexpr int B = (B0 / NP) > 2 ? (B0 / NP) : 2;
   const bool can_plan = (K + 2 <= kOC);              // next-step outputs fit one register chunk
-  int tab_log2 = 0;  // largest power-of-two table in the staged-output + coefficient areas
+  int tab_log2 = 0;  // largest power-of-two word count in the staged-output + coefficient areas
   while ((2ull << tab_log2) <= (unsigned long long)KO * OST + cmat_floats(K, W)) tab_log2++;
   // The BF16-MFMA and FP8 forms run GEMM1 on MFMA where the shapes allow.
   const bool use_mfma = !std::is_same<R, float>::value && a.mfma && mfma_ok(D, K, W);
@@ -1146,8 +1153,8 @@ __global__ __launch_bounds__(64 * kMaxWpb) void k_train(TrainArgs a) {
       wave_sync();
       // The staged-output and coefficient areas (contiguous, free here) hold
       // the hash table.
-      if (a.V <= (1 << 21) && 4 * NN <= 3 * (1 << tab_log2))
-        first_occurrence_hash(s.negw, s.negs, NN, lane, reinterpret_cast<unsigned*>(s.O), tab_log2);
+      if (tab_log2 >= 2 && 4 * NN <= 3 * (1 << (tab_log2 - 1)))
+        first_occurrence_hash(s.negw, s.negs, NN, lane, reinterpret_cast<unsigned*>(s.O), tab_log2 - 1);
       else
         first_occurrence(s.negw, s.negs, NN, lane);
       for (int w = lane; w < kBloomWords; w += 64) s.nbloom[w] = 0u;

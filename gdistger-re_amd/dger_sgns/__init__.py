@@ -269,20 +269,15 @@ def walk_device(rowptr, col, cnb, opts: WalkOpts):
 
 
 def walk(rowptr, col, cnb, opts: WalkOpts):
-    """Information-oriented walks on the GPU (dger_walk_run).  Returns the
-    corpus (vertex ids, -1 after every walk), vertex_cn and the stats."""
-    rowptr = np.ascontiguousarray(rowptr, np.int64)
-    col = _i32arr(col)
-    cnb = _i32arr(cnb)
-    V = rowptr.size - 1
-    cap = int(opts.n_walkers) * (int(opts.max_length) + 1)
-    corpus = np.empty(max(cap, 1), np.int32)
-    vcn = np.zeros(V, np.int32)
-    n = C.c_uint64(0)
-    st = WalkStats()
-    _check(lib().dger_walk_run(_ptr(rowptr), _ptr(col), _ptr(cnb), V, C.byref(opts), _ptr(corpus), cap,
-                               C.byref(n), _ptr(vcn), C.byref(st)))
-    return corpus[:n.value].copy(), vcn, st
+    """Information-oriented walks on the GPU.  Returns the corpus (vertex
+    ids, -1 after every walk), vertex_cn and the stats.  The walks are made
+    in device memory (dger_walk_run_device) and copied out at their real
+    length, so the host never holds a buffer sized for the longest possible
+    corpus (n_walkers * (max_length + 1) ids)."""
+    wc, vcn, st = walk_device(rowptr, col, cnb, opts)
+    with wc:
+        corpus = wc.download()
+    return corpus, vcn, st
 
 
 class Trainer:

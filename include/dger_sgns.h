@@ -210,7 +210,8 @@ int dger_dev_zero(void *ptr, uint64_t bytes);
 /* Link prediction on the context's current model (north-star evaluation;
    held-out split as src/tools/gtrain.cpp:300-343).  pairs: n x 2 vertex ids.
    dger_ctx_link_scores: scores[i] = dot(Wih[u_i], Wih[v_i]) computed on the
-   device (0 for ids outside the vocabulary).  dger_ctx_link_auc: ROC AUC of
+   device.  Every id must lie in [0, v_num): a pair outside it fails the call
+   with DGER_E_ARG before anything is uploaded.  dger_ctx_link_auc: ROC AUC of
    those scores against labels (1 = held-out edge, 0 = non-edge) by the
    Mann-Whitney statistic with ties counting one half, computed on the device;
    NaN when a class is empty. */

@@ -46,6 +46,12 @@ CASES = [
     (256, 2, 15, 1e-3, 1, dict(seed=5, n_walks=300)),
     (32, 1, 0, 1e-3, 1, dict(seed=6)),
     (128, 10, 5, 1e-3, 1, dict(seed=7, n_walks=120, long_walk=2600)),
+    # long sentences at large D: a 600-token walk gives 3000 negative draws per
+    # batch, past the 2048 positions an 11-bit position field could hold; at
+    # D=1024 the first-draw hash table has 4096 slots and takes them, at
+    # D=512 the batch is over 3/4 of the table and takes the scan instead
+    (512, 5, 5, 1e-3, 1, dict(seed=13, n_walks=150, long_walk=600)),
+    (1024, 2, 5, 1e-3, 1, dict(seed=14, n_walks=150, long_walk=600)),
 ]
 
 
