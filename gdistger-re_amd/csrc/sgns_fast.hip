@@ -1,0 +1,938 @@
+// gfx950 window-slot trainer of the BF16-MFMA form (DistGER DSGL,
+// include/dsgl.hpp:786-1408).
+//
+// One 64-lane wavefront per Hogwild worker, as in sgns_kernels.hip: worker k
+// trains the corpus slice and LCG stream of the reference's compute thread
+// k+1, so the index stream (sentences, subsampling, windows, negative groups,
+// output lists, learning rate) is the reference's.  What differs is where the
+// rows live and how a step is computed:
+//
+//   * Window slots.  The FP32 working row of every rank inside a sentence's
+//     window [p-W, p+W] sits in registers ("slots", 24 per sentence).  A row
+//     is read from Wih when its rank enters the window and written back to
+//     Wih (bf16, stochastic rounding) when its last occurrence leaves it or
+//     the sentence ends.  The reference keeps the whole sentence in an FP32
+//     buffer and writes it back at the sentence end (dsgl.hpp:959-962,
+//     :1377-1395); here a rank that leaves the window and comes back later
+//     in the sentence is read again from Wih.
+//   * Direct output updates.  Target and negative rows take their updates
+//     straight in Woh (bf16 atomic adds).  The reference adds target updates
+//     to Woh directly too (dsgl.hpp:1352-1356) but buffers negatives per
+//     sentence pair and writes the buffer back over Woh (dsgl.hpp:1081-1084,
+//     :1358-1366, :1396-1404).
+//   * Repeated contexts of a step share their slot; their coefficients are
+//     equal (same row, same label), so the slot takes one coefficient scaled
+//     by the repeat count instead of the reference's sequential adds
+//     (dsgl.hpp:1314-1333).
+//   * The three contractions of a step (dsgl.hpp:1241, :1290, :1309) run on
+//     MFMA with bf16 operands and FP32 accumulation, over all 48 slots (a
+//     slot that is no context of the step has coefficient 0):
+//       GEMM1  F^T = O I^T         v_mfma_f32_16x16x32_bf16, 3 x 4 per step
+//       GEMM2  md  = C^T I         v_mfma_f32_16x16x32_bf16, 8 x 2
+//       GEMM3  I  += C O           v_mfma_f32_16x16x16_bf16, 3 x 8, the
+//                                  slots' registers as the accumulator.
+//
+// Register layout (D = 128, lane = 16 g + m):
+//   slot s = 16 rt + 4 g' + r lives in the lanes of group g' = (s >> 2) & 3;
+//   lane (g', m) holds elements 8m .. 8m+7 of it in R[rt][t][r], t = 0..7.
+// This is the output layout of GEMM3 (16x16x16: D[4g+r][n] for rows = slots,
+// columns = elements 8n + t of tile t) and the B operand layout of GEMM2 (the
+// K index 8g + e of block kb is slot 16 (2kb + e/4) + 4g + e%4).  GEMM1 needs
+// the slots across lanes instead; it reads a bf16 copy of the slots that each
+// step writes to LDS.
+//
+// Output rows (the step's target, distinct negatives, second target; at most
+// 16) are read from Woh in two layouts: for GEMM1's A operand lane (g, m)
+// reads row m, elements 32 ks + 8 g .. +8 (ks = 0..3); for GEMM3's B operand
+// lane (g, m) reads rows 4g .. 4g+3, elements 8m .. 8m+7.  The next step's
+// rows are read right after this step's updates (so they include them).
+
+#include <hip/hip_runtime.h>
+
+#include "dger_sgns.h"
+#include "sgns_internal.h"
+
+namespace dger {
+namespace {
+
+typedef unsigned long long u64;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int D = kFastDim;           // row length (128)
+constexpr int NT = D / 16;            // element tiles of GEMM2 / GEMM3 (8)
+constexpr int KB1 = D / 32;           // K blocks of GEMM1 (4)
+constexpr int kSlots = 48;            // 3 MFMA tiles of 16 slots
+constexpr int kSentSlots = 24;        // slots of one sentence
+constexpr int kOutRows = 16;          // output rows per step (MFMA tile)
+constexpr int RCS = 2 * D + 16;       // bytes per slot row of the LDS bf16 copy (padded against bank conflicts)
+constexpr int CBS = 144;              // bytes per output row of the transposed coefficients
+constexpr int kNone = 1 << 30;
+constexpr unsigned kRsrcWord3 = 0x00020000u;  // raw buffer descriptor word 3 for gfx9
+constexpr int kSc1 = 16;                       // cache-policy bit: coherent across XCDs (agent scope)
+
+static_assert(D % 32 == 0 && NT == 8, "the slot layout is written for D = 128");
+
+__device__ __forceinline__ u64 readlane64(u64 v, int l) {
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), l);
+  return ((u64)hi << 32) | lo;
+}
+__device__ __forceinline__ u64 lanemask_lt(int lane) { return (1ull << lane) - 1ull; }
+
+__device__ __forceinline__ unsigned hash32(unsigned x) {
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+
+// Two floats to a bf16 pair, round to nearest even.
+__device__ __forceinline__ unsigned pack_rne(float a, float b) {
+  bf16x2 v;
+  v[0] = (__bf16)a;
+  v[1] = (__bf16)b;
+  return __builtin_bit_cast(unsigned, v);
+}
+// Stochastic rounding of finite x to bf16 with 16 random bits.
+__device__ __forceinline__ unsigned bf16_sr(float x, unsigned rnd) {
+  return (__float_as_uint(x) + (rnd & 0xFFFFu)) >> 16;
+}
+__device__ __forceinline__ void unpack8(const u32x4& v, float (&e)[8]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    e[2 * k] = __uint_as_float(v[k] << 16);
+    e[2 * k + 1] = __uint_as_float(v[k] & 0xFFFF0000u);
+  }
+}
+
+// Table-quantised sigmoid gradient (dsgl.hpp:1249-1270); the index is
+// clamped only so that an undefined input cannot read out of bounds.
+__device__ __forceinline__ float sig_grad(float f, int label, float alpha, const float* exp_table) {
+  int idx = (int)((f + 6.0f) * 83.0f);
+  idx = min(max(idx, 0), kExpTable - 1);
+  const float e = exp_table[idx];
+  const float lab = (float)label;
+  const float mid = (lab - e) * alpha;
+  const float hi = (lab - 1.0f) * alpha;
+  const float lo = lab * alpha;
+  return f > 6.0f ? hi : (f < -6.0f ? lo : mid);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// 16-byte row-segment loads and stores at agent scope through a raw buffer
+// descriptor (the matrices stay below 4 GiB; checked on the host).
+struct Mat {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ Mat(const void* base, unsigned bytes)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, (int)kRsrcWord3)) {}
+  __device__ __forceinline__ u32x4 load(unsigned off) const {
+    return __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, kSc1);
+  }
+  __device__ __forceinline__ void store(unsigned off, const u32x4& v) const {
+    __builtin_amdgcn_raw_buffer_store_b128(v, rs, (int)off, 0, kSc1);
+  }
+  // 4 bytes per lane straight into LDS at dst + 4 * lane (dst wave-uniform);
+  // the data is there after wait_vm_all().
+  __device__ __forceinline__ void load_lds(unsigned off, unsigned char* dst) const {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)dst, 4, off, 0, 0, kSc1);
+  }
+};
+
+// Waits for every vector memory access of the wave (s_waitcnt vmcnt(0)),
+// then orders the LDS accesses after it.
+__device__ __forceinline__ void wait_vm_all() {
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// For each position p < n: first, previous and next position holding the
+// same value (kNone / -1 where there is none).
+__device__ inline void occurrences(const int* v, int* first, int* prev, int* next, int n, int lane) {
+  for (int c0 = 0; c0 < n; c0 += 64) {
+    const int p = c0 + lane;
+    const int x = (p < n) ? v[p] : 0;
+    int f = p, pv = -1, nx = kNone;
+#pragma unroll 8
+    for (int q = 0; q < n; q++) {
+      const bool eq = v[q] == x;
+      f = (eq && q < f) ? q : f;
+      pv = (eq && q < p) ? q : pv;
+      nx = (eq && q > p && nx == kNone) ? q : nx;
+    }
+    if (p < n) { first[p] = f; prev[p] = pv; next[p] = nx; }
+  }
+}
+
+// ---- the slots' registers -------------------------------------------------------
+struct Slots {
+  f32x4 R[3][NT];  // R[rt][t][r]: element 8m + t of slot 16 rt + 4 g + r
+};
+
+// Slot s (wave-uniform) takes the row whose elements 8m .. 8m+7 this lane
+// holds in v; only the owner group's lanes change their registers.
+__device__ __forceinline__ void slot_put(Slots& S, int s, const u32x4& v, int g) {
+  float e[8];
+  unpack8(v, e);
+  const bool own = g == ((s >> 2) & 3);
+  switch (((s >> 4) << 2) | (s & 3)) {
+#define DGER_PUT(RT, RR)                                             \
+  case RT * 4 + RR:                                                  \
+    _Pragma("unroll") for (int t = 0; t < NT; t++) S.R[RT][t][RR] = own ? e[t] : S.R[RT][t][RR]; \
+    break;
+    DGER_PUT(0, 0) DGER_PUT(0, 1) DGER_PUT(0, 2) DGER_PUT(0, 3)
+    DGER_PUT(1, 0) DGER_PUT(1, 1) DGER_PUT(1, 2) DGER_PUT(1, 3)
+    DGER_PUT(2, 0) DGER_PUT(2, 1) DGER_PUT(2, 2) DGER_PUT(2, 3)
+#undef DGER_PUT
+    default: break;
+  }
+}
+
+// Elements 8m .. 8m+7 of slot s held by this lane (meaningful in the owner group).
+__device__ __forceinline__ void slot_get(const Slots& S, int s, float (&e)[8]) {
+  switch (((s >> 4) << 2) | (s & 3)) {
+#define DGER_GET(RT, RR)                                                     \
+  case RT * 4 + RR:                                                          \
+    _Pragma("unroll") for (int t = 0; t < NT; t++) e[t] = S.R[RT][t][RR];     \
+    break;
+    DGER_GET(0, 0) DGER_GET(0, 1) DGER_GET(0, 2) DGER_GET(0, 3)
+    DGER_GET(1, 0) DGER_GET(1, 1) DGER_GET(1, 2) DGER_GET(1, 3)
+    DGER_GET(2, 0) DGER_GET(2, 1) DGER_GET(2, 2) DGER_GET(2, 3)
+#undef DGER_GET
+    default:
+      _Pragma("unroll") for (int t = 0; t < NT; t++) e[t] = 0.f;
+      break;
+  }
+}
+
+// Eight floats to bf16 with stochastic rounding (random bits from seed).
+__device__ __forceinline__ u32x4 pack8_sr(const float (&e)[8], unsigned seed) {
+  u32x4 w;
+  unsigned h = hash32(seed);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    w[k] = bf16_sr(e[2 * k], h) | (bf16_sr(e[2 * k + 1], h >> 16) << 16);
+    h ^= h << 13; h ^= h >> 17; h ^= h << 5;
+  }
+  return w;
+}
+
+// ---- one step's contractions ----------------------------------------------------
+// LDS of the step: rc (bf16 slot rows, RCS bytes each), cb (coefficients
+// transposed for GEMM2: [output][owner group][16 slots in K order]).
+struct StepLds {
+  unsigned char* rc;
+  unsigned char* cb;
+  const int* mult;  // contexts of the step per slot
+};
+
+// bf16 copy of every slot row into rc (GEMM1 reads the slots across lanes).
+__device__ __forceinline__ void ring_copy(const Slots& S, unsigned char* rc, int g, int m) {
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      u32x4 v;
+#pragma unroll
+      for (int k = 0; k < 4; k++) v[k] = pack_rne(S.R[rt][2 * k][r], S.R[rt][2 * k + 1][r]);
+      *reinterpret_cast<u32x4*>(rc + (16 * rt + 4 * g + r) * RCS + 16 * m) = v;
+    }
+}
+
+// GEMM1 + sigmoid + coefficients.  o1[ks]: row m of the outputs, elements
+// 32 ks + 8 g .. +8.  On return C[rt][r] = coefficient of slot 16 rt + m and
+// output 4 g + r (0 unless the slot is a context and the output exists), cb
+// holds the coefficients for GEMM2.  meta4[r] = multiplicity of output 4g+r
+// (0 past the step's outputs); os = number of outputs.
+__device__ __forceinline__ void gemm1_coef(const u32x4 (&o1)[KB1], const StepLds& L, const int (&meta4)[4], int os,
+                                           float alpha, const float* expt, int g, int m, float (&C)[3][4]) {
+  f32x4 f[3];
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++) f[rt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < KB1; ks++) {
+    const bf16x8 a = __builtin_bit_cast(bf16x8, o1[ks]);
+#pragma unroll
+    for (int rt = 0; rt < 3; rt++) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(L.rc + (16 * rt + m) * RCS + (32 * ks + 8 * g) * 2);
+      f[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, f[rt], 0, 0, 0);
+    }
+  }
+  // f[rt][r] = F[slot 16 rt + m][output 4 g + r]
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++) {
+    const int sl = 16 * rt + m;
+    const int mu = L.mult[sl];
+    const bool second = sl >= kSentSlots;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = 4 * g + r;
+      const int label = second ? (i == os - 1 ? 1 : 0) : (i == 0 ? 1 : 0);
+      const int cnt = meta4[r] * mu;
+      const float gr = sig_grad(f[rt][r], label, alpha, expt);
+      C[rt][r] = cnt ? gr * (float)cnt : 0.f;
+      *reinterpret_cast<__bf16*>(L.cb + i * CBS + (m >> 2) * 32 + (4 * rt + (m & 3)) * 2) = (__bf16)C[rt][r];
+    }
+  }
+}
+
+// GEMM2: md[4g + r][8m + t] = acc[t][r], from the coefficients in cb and the
+// slots' values before this step's update.
+__device__ __forceinline__ void gemm2(const Slots& S, const StepLds& L, int g, int m, f32x4 (&acc)[NT]) {
+  const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(L.cb + m * CBS + g * 32);
+  const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(L.cb + m * CBS + g * 32 + 16);
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    u32x4 b0, b1;
+    b0[0] = pack_rne(S.R[0][t][0], S.R[0][t][1]);
+    b0[1] = pack_rne(S.R[0][t][2], S.R[0][t][3]);
+    b0[2] = pack_rne(S.R[1][t][0], S.R[1][t][1]);
+    b0[3] = pack_rne(S.R[1][t][2], S.R[1][t][3]);
+    b1[0] = pack_rne(S.R[2][t][0], S.R[2][t][1]);
+    b1[1] = pack_rne(S.R[2][t][2], S.R[2][t][3]);
+    b1[2] = 0u;
+    b1[3] = 0u;
+    f32x4 c = (f32x4){0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, __builtin_bit_cast(bf16x8, b0), c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, __builtin_bit_cast(bf16x8, b1), c, 0, 0, 0);
+    acc[t] = c;
+  }
+}
+
+// GEMM3: every slot row += sum over outputs of C[slot][i] O[i] (zero for
+// slots that are no context).  o3[e]: row 4g + e of the outputs, elements
+// 8m .. 8m+7.
+__device__ __forceinline__ void gemm3(Slots& S, const float (&C)[3][4], const u32x4 (&o3)[4]) {
+  short4v a[3];
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++) {
+    const unsigned lo = pack_rne(C[rt][0], C[rt][1]), hi = pack_rne(C[rt][2], C[rt][3]);
+    a[rt] = __builtin_bit_cast(short4v, (u64)lo | ((u64)hi << 32));
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const unsigned sel = (t & 1) ? 0x07060302u : 0x05040100u;
+    const unsigned lo = __builtin_amdgcn_perm(o3[1][t >> 1], o3[0][t >> 1], sel);
+    const unsigned hi = __builtin_amdgcn_perm(o3[3][t >> 1], o3[2][t >> 1], sel);
+    const short4v b = __builtin_bit_cast(short4v, (u64)lo | ((u64)hi << 32));
+#pragma unroll
+    for (int rt = 0; rt < 3; rt++) S.R[rt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[rt], b, S.R[rt][t], 0, 0, 0);
+  }
+}
+
+// LDS carve-up of one worker of the window-slot trainer (16-byte aligned).
+struct FastLds {
+  unsigned char* rc;  // kSlots x RCS
+  unsigned char* cb;  // kOutRows x CBS
+  int* mult;          // kSlots
+  int* srank;         // kSlots: rank held by each slot
+  int* plans;         // two plans: ranks [0, 16), multiplicities [16, 32)
+  int* tok;           // 2 x Lmax ranks
+  int* prv;           // 2 x Lmax
+  int* nxt;           // 2 x Lmax
+  int* sop;           // 2 x Lmax: slot of each window position
+  int* negw;          // K x Lmax negative draws
+};
+
+__host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t fast_layout(int Lmax, int K, size_t* off) {
+  size_t o = 0;
+  off[0] = o; o = al16(o + (size_t)kSlots * RCS);
+  off[1] = o; o = al16(o + (size_t)kOutRows * CBS);
+  off[2] = o; o = al16(o + 2 * (size_t)kSlots * 4);
+  off[3] = o; o = al16(o + 2 * 2 * (size_t)kOutRows * 4);
+  off[4] = o; o = al16(o + 8 * (size_t)Lmax * 4);
+  off[5] = o; o = al16(o + (size_t)(K > 0 ? K : 1) * Lmax * 4);
+  return o;
+}
+
+__device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K) {
+  size_t off[6];
+  fast_layout(Lmax, K, off);
+  FastLds s;
+  s.rc = base + off[0];
+  s.cb = base + off[1];
+  s.mult = reinterpret_cast<int*>(base + off[2]);
+  s.srank = s.mult + kSlots;
+  s.plans = reinterpret_cast<int*>(base + off[3]);
+  int* b4 = reinterpret_cast<int*>(base + off[4]);
+  s.tok = b4;
+  s.prv = b4 + 2 * Lmax;
+  s.nxt = b4 + 4 * Lmax;
+  s.sop = b4 + 6 * Lmax;
+  s.negw = reinterpret_cast<int*>(base + off[5]);
+  return s;
+}
+
+__host__ __device__ inline size_t exp_bytes() { return al16((size_t)kExpTable * 4); }
+
+// Output rows of plan pl into the two register layouts (see the file comment);
+// rows past os read row 0 of the plan (their results are not used).
+__device__ __forceinline__ void load_outputs(const Mat& woh, const int* plan, int os, int g, int m, u32x4 (&o1)[KB1],
+                                             u32x4 (&o3)[4]) {
+  const int r1 = plan[m < os ? m : 0];
+#pragma unroll
+  for (int ks = 0; ks < KB1; ks++) o1[ks] = woh.load((unsigned)r1 * (D * 2) + (32 * ks + 8 * g) * 2);
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const int i = 4 * g + e;
+    const int r3 = plan[i < os ? i : 0];
+    o3[e] = woh.load((unsigned)r3 * (D * 2) + 16 * m);
+  }
+}
+
+__device__ __forceinline__ void load_output_l1(const Mat& woh, const int* plan, int os, int g, int m, u32x4 (&o1)[KB1]) {
+  const int r1 = plan[m < os ? m : 0];
+#pragma unroll
+  for (int ks = 0; ks < KB1; ks++) o1[ks] = woh.load((unsigned)r1 * (D * 2) + (32 * ks + 8 * g) * 2);
+}
+
+__device__ __forceinline__ void load_output_l3(const Mat& woh, const int* plan, int os, int g, int m, u32x4 (&o3)[4]) {
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const int i = 4 * g + e;
+    const int r3 = plan[i < os ? i : 0];
+    o3[e] = woh.load((unsigned)r3 * (D * 2) + 16 * m);
+  }
+}
+
+// Woh rows of the outputs += md (bf16 atomic adds of element pairs).
+__device__ __forceinline__ void scatter_outputs(void* woh, const int* plan, int os, int g, int m, const f32x4 (&md)[NT]) {
+  typedef short __attribute__((ext_vector_type(2))) short2v;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int i = 4 * g + r;
+    if (i < os) {
+      uint16_t* row = static_cast<uint16_t*>(woh) + (size_t)plan[i] * D + 8 * m;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const unsigned w = pack_rne(md[2 * k][r], md[2 * k + 1][r]);
+        (void)__builtin_amdgcn_flat_atomic_fadd_v2bf16(reinterpret_cast<short2v*>(row) + k,
+                                                       __builtin_bit_cast(short2v, w));
+      }
+    }
+  }
+}
+
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, m = lane & 15;
+  const int wave = threadIdx.x >> 6;
+  const int wk = blockIdx.x * a.wpb + wave;
+  const int W = a.W, K = a.K, Lmax = a.Lmax;
+  float* const expt = reinterpret_cast<float*>(smem);
+  for (int x = threadIdx.x; x < kExpTable; x += blockDim.x) expt[x] = a.exp_table[x];
+  __syncthreads();
+  if (wk >= a.workers) return;
+  size_t off_unused[6];
+  const FastLds s = fast_carve(smem + exp_bytes() + (size_t)wave * fast_layout(Lmax, K, off_unused), Lmax, K);
+  const Mat wih(a.wih, (unsigned)((size_t)a.V * D * 2)), woh(a.woh, (unsigned)((size_t)a.V * D * 2));
+
+  // Slot copies and coefficients start at zero (unused slots and the K
+  // padding of GEMM2 must multiply as finite zeros).
+  for (int x = lane; x < (kSlots * RCS) / 16; x += 64) reinterpret_cast<u32x4*>(s.rc)[x] = (u32x4){0u, 0u, 0u, 0u};
+  for (int x = lane; x < (kOutRows * CBS) / 16; x += 64) reinterpret_cast<u32x4*>(s.cb)[x] = (u32x4){0u, 0u, 0u, 0u};
+  if (lane < kSlots) { s.mult[lane] = 0; s.srank[lane] = 0; }
+  if (lane < 2 * 2 * kOutRows) s.plans[lane] = 0;
+  Slots S;
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) S.R[rt][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  wave_sync();
+
+  // Corpus slice and seed of compute thread wk+1 (dsgl.hpp:791, :811-821).
+  const u64 N = a.n_tokens;
+  const u64 nparts = (u64)a.workers;
+  const u64 per = N / nparts;
+  const u64 start = per * (u64)wk;
+  u64 cnt_tok = per;
+  if (N % nparts > 0 && wk == a.workers - 1) cnt_tok = N - (nparts - 1) * per;
+  const u64 end = start + cnt_tok;
+  u64 rng = (u64)wk;
+
+  // Lane l holds the (l+1)-step LCG map s -> jA*s + jC.
+  u64 jA = 1, jC = 0;
+  for (int i = 0; i <= lane; i++) { jA = jA * kLcgMul; jC = jC * kLcgMul + kLcgAdd; }
+
+  const float a0 = a.alpha0;
+  float alpha = a0;
+  int iters_left = a.iter;
+  u64 wc = 0, last_wc = 0, cur = start;
+  int len[2] = {0, 0}, pos[2] = {0, 0};
+  unsigned freem[2] = {0xFFFFFFu, 0xFFFFFFu};  // free slots of each sentence (bit k: slot 24q + k)
+  int done_sent = 0, refresh = 1, neg_count = 0, nbn = 1;
+  u64 steps = 0, raw = 0, sentences = 0, n_ctx = 0, n_outctx = 0;
+  const int stride = record_stride(W, K);
+  int32_t* const rec = a.rec ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
+  int plan = 0;
+  bool planned = false;
+  int pre_os = 0;
+  u32x4 o1[KB1], o3[4];
+  u32x4 in_row[2];          // rows entering the windows at this step's transition
+  unsigned salt = hash32((unsigned)wk * 0x9E3779B9u + 0x6A09E667u);
+
+  // Learning-rate decay, sentence loads, slice end, refresh: as the reference
+  // loop (dsgl.hpp:886-1088).
+  for (;;) {
+    // Lane-derived values are recomputed in every iteration from an opaque
+    // copy of the lane index, instead of being kept live across the loop.
+    int lane_o = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane_o));
+    const int lane = lane_o, g = lane_o >> 4, m = lane_o & 15;
+    if (wc - last_wc > 10000) {
+      const u64 diff = wc - last_wc;
+      u64 old = 0;
+      if (lane == 0) old = atomicAdd(&a.counters[kWordCountActual], diff);
+      const u64 total = readlane64(old, 0) + diff;
+      last_wc = wc;
+      alpha = a0 * (1 - (float)total / a.alpha_den);
+      if (alpha < a0 * 0.0001f) alpha = a0 * 0.0001f;
+    }
+
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      int* const tq = s.tok + q * Lmax;
+      for (;;) {
+        if (len[q] != 0) break;
+        if (cur >= end) break;
+        int L = 0;
+        for (;;) {
+          const u64 idx = cur + (u64)lane;
+          const int id = (idx < end) ? a.corpus[idx] : -1;
+          const bool term = (id == -1) || (idx + 1 >= end);
+          const u64 tm = __ballot(term);
+          const int tl = tm ? (__ffsll((long long)tm) - 1) : 64;
+          const bool regular = lane < tl;
+          int rk = 0;
+          bool keep = regular;
+          if (regular) {
+            rk = a.id_rank[id];
+            if (a.sample_on) {
+              const u64 sr = jA * rng + jC;
+              keep = !(a.keep_thr[rk] < (float)(unsigned)(sr & 0xFFFFu) / 65536.f);
+            }
+          }
+          const u64 km = __ballot(keep);
+          int nk = __popcll(km);
+          const int room = kMaxSentence - L;
+          int consumed, draws;
+          bool stop;
+          if (nk >= room) {
+            const u64 cm = __ballot(keep && (__popcll(km & lanemask_lt(lane)) + 1 == room));
+            const int cl = __ffsll((long long)cm) - 1;
+            keep = keep && (lane <= cl);
+            consumed = cl + 1; draws = cl + 1; stop = true; nk = room;
+          } else if (tl < 64) {
+            consumed = tl + 1; draws = tl; stop = true;
+          } else {
+            consumed = 64; draws = 64; stop = false;
+          }
+          if (keep) tq[L + __popcll(km & lanemask_lt(lane))] = rk;
+          L += nk;
+          if (a.sample_on && draws > 0) rng = readlane64(jA, draws - 1) * rng + readlane64(jC, draws - 1);
+          cur += (u64)consumed;
+          wc += (u64)consumed;
+          if (stop) break;
+        }
+        pos[q] = 0;
+        len[q] = L > 1 ? L : 0;
+        if (L <= 1) continue;
+        wave_sync();
+        int* const sq = s.sop + q * Lmax;
+        occurrences(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, lane);
+        wave_sync();
+        // Slots of the first window [0, W]: the k-th distinct rank takes slot 24q + k.
+        const int nw = min(W, L - 1) + 1;
+        const bool isnew = lane < nw && s.prv[q * Lmax + lane] < 0;
+        const u64 nm = __ballot(isnew);
+        const int nnew = __popcll(nm);
+        int myslot = 0;
+        if (lane < nw) myslot = kSentSlots * q + __popcll(nm & lanemask_lt(sq[lane]));
+        wave_sync();
+        if (lane < nw) sq[lane] = myslot;
+        if (isnew) s.srank[myslot] = tq[lane];
+        freem[q] = 0xFFFFFFu & ~((1u << nnew) - 1u);
+        wave_sync();
+        // Read the new slots' rows into LDS (the bf16 slot copy is rewritten
+        // at the next step, so it serves as the staging area), then move
+        // them into their slots.
+        for (int k = 0; k < nnew; k++) {
+          const int rk = __builtin_amdgcn_readfirstlane(s.srank[kSentSlots * q + k]);
+          wih.load_lds((unsigned)rk * (D * 2) + 4 * lane, s.rc + k * (D * 2));
+        }
+        wait_vm_all();
+        for (int k = 0; k < nnew; k++)
+          slot_put(S, kSentSlots * q + k, *reinterpret_cast<const u32x4*>(s.rc + k * (D * 2) + 16 * m), g);
+        wave_sync();
+        planned = false;
+      }
+    }
+
+    // End of the slice: in-flight sentences are dropped (dsgl.hpp:1031-1050).
+    if (cur >= end) {
+      if (lane == 0) atomicAdd(&a.counters[kWordCountActual], wc - last_wc);
+      raw += wc;
+      if (--iters_left == 0) break;
+      wc = 0; cur = start; last_wc = 0;
+      len[0] = 0; len[1] = 0;
+      freem[0] = 0xFFFFFFu; freem[1] = 0xFFFFFFu;
+      planned = false;
+      continue;
+    }
+
+    // Shared negative batch (dsgl.hpp:1054-1080): K * max(L1, L2) draws.
+    if (refresh) {
+      nbn = max(len[0], len[1]);
+      const int NN = nbn * K;
+      const u64 A64 = readlane64(jA, 63), C64 = readlane64(jC, 63);
+      const u64 r0 = rng;
+      u64 cs = r0;
+      for (int base = 0; base < NN; base += 64 * 8) {
+        u64 sr[8];
+        int v[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          sr[c] = jA * cs + jC;
+          cs = A64 * cs + C64;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const int n = base + 64 * c + lane;
+          v[c] = (n < NN) ? a.table[(sr[c] >> 16) % a.table_size] : 1;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const int n = base + 64 * c + lane;
+          if (n < NN) s.negw[n] = v[c] ? v[c] : (int)(sr[c] % (u64)(a.V - 1) + 1);
+        }
+      }
+      {
+        u64 f = r0;
+        int left = NN;
+        while (left >= 64) { f = A64 * f + C64; left -= 64; }
+        if (left > 0) f = readlane64(jA, left - 1) * f + readlane64(jC, left - 1);
+        rng = f;
+      }
+      refresh = 0;
+      planned = false;
+      wave_sync();
+    }
+
+    // ---- one step (dsgl.hpp:1089-1368) ----
+    const int p0 = pos[0], p1 = pos[1];
+    const int t1 = __builtin_amdgcn_readfirstlane(s.tok[p0]);
+    const int t2 = __builtin_amdgcn_readfirstlane(s.tok[Lmax + p1]);
+    rng = rng * kLcgMul + kLcgAdd;
+    const int b = (int)(rng % (u64)W);
+    const int hw = W - b;
+    const int lo0 = max(0, p0 - hw), hi0 = min(len[0] - 1, p0 + hw);
+    const int lo1 = max(0, p1 - hw), hi1 = min(len[1] - 1, p1 + hw);
+    const int n0 = (p0 - lo0) + (hi0 - p0);
+    const int n1 = (p1 - lo1) + (hi1 - p1);
+    const int nin = n0 + n1, fin = n0;
+
+    // Rows entering the windows at this step's transition: read now.
+    int in_slot_src[2];  // position whose slot the entering position shares, or -1 (new row)
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int pq = q ? p1 : p0, Lq = len[q];
+      const int x = pq - W, y = pq + W + 1;
+      in_slot_src[q] = -2;  // nothing enters
+      if (pq + 1 < Lq && y < Lq) {
+        const int pv = __builtin_amdgcn_readfirstlane(s.prv[q * Lmax + y]);
+        in_slot_src[q] = (pv >= 0 && pv >= x) ? pv : -1;
+      }
+      const int rk = (in_slot_src[q] == -1) ? __builtin_amdgcn_readfirstlane(s.tok[q * Lmax + y]) : 0;
+      in_row[q] = wih.load((unsigned)rk * (D * 2) + 16 * m);
+    }
+
+    // Context multiplicity per slot.
+    if (lane < kSlots) s.mult[lane] = 0;
+    wave_sync();
+    int my_ctx_rank = -1;
+    if (lane < nin) {
+      const int q = (lane < n0) ? 0 : 1;
+      const int jj = q ? lane - n0 : lane;
+      const int lo = q ? lo1 : lo0, pp = q ? p1 : p0;
+      const int c = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
+      atomicAdd(&s.mult[s.sop[q * Lmax + c]], 1);
+      my_ctx_rank = s.tok[q * Lmax + c];
+    }
+
+    // Output list (dsgl.hpp:1136-1158): target, distinct negatives of the
+    // group with multiplicities (a negative equal to the target is dropped),
+    // second target; rows past the count have multiplicity 0.
+    neg_count++;
+    const int grp = neg_count % nbn;
+    auto make_outputs = [&](int pl, int ta, int tb, int gq) -> int {
+      int* const pr = s.plans + pl * 2 * kOutRows;
+      const bool live = lane < K;
+      const int v = live ? s.negw[gq * K + lane] : -1;
+      int first = lane, mult = 0;
+      for (int k2 = 0; k2 < K; k2++) {
+        const int w2 = __builtin_amdgcn_readlane(v, k2);
+        mult += (w2 == v) ? 1 : 0;
+        first = (w2 == v && k2 < first) ? k2 : first;
+      }
+      const bool keep = live && first == lane && v != ta;
+      const u64 km = __ballot(keep);
+      const int o = 1 + __popcll(km & lanemask_lt(lane));
+      const int n = 2 + __popcll(km);
+      if (lane < kOutRows) pr[kOutRows + lane] = 0;
+      wave_sync();
+      if (keep) { pr[o] = v; pr[kOutRows + o] = mult; }
+      if (lane == 0) { pr[0] = ta; pr[kOutRows] = 1; pr[n - 1] = tb; pr[kOutRows + n - 1] = 1; }
+      wave_sync();
+      return n;
+    };
+    int os;
+    if (planned) {
+      os = pre_os;
+    } else {
+      os = make_outputs(plan, t1, t2, grp);
+      load_outputs(woh, s.plans + plan * 2 * kOutRows, os, g, m, o1, o3);
+    }
+    const int* const cplan = s.plans + plan * 2 * kOutRows;
+
+    if (rec && (int64_t)steps < a.rec_cap) {
+      int32_t* r = rec + (size_t)steps * stride;
+      for (int z = lane; z < stride; z += 64) r[z] = -1;
+      wave_sync();
+      if (lane == 0) {
+        r[0] = t1; r[1] = t2; r[2] = b; r[3] = fin; r[4] = nin; r[5] = os;
+        r[6] = __float_as_int(alpha);
+        r[7] = grp;
+      }
+      if (lane < nin) r[8 + lane] = my_ctx_rank;
+      if (lane < os) { r[8 + 4 * W + lane] = cplan[lane]; r[8 + 4 * W + K + 2 + lane] = cplan[kOutRows + lane]; }
+    }
+
+    // Contractions.
+    ring_copy(S, s.rc, g, m);
+    int meta4[4];
+    {
+      const int4 mm = *reinterpret_cast<const int4*>(cplan + kOutRows + 4 * g);
+      meta4[0] = mm.x; meta4[1] = mm.y; meta4[2] = mm.z; meta4[3] = mm.w;
+    }
+    wave_sync();
+    const StepLds L{s.rc, s.cb, s.mult};
+    float C[3][4];
+    gemm1_coef(o1, L, meta4, os, alpha, expt, g, m, C);
+    wave_sync();
+    {
+      f32x4 md[NT];
+      gemm2(S, L, g, m, md);
+      scatter_outputs(a.woh, cplan, os, g, m, md);
+    }
+
+    // Plan the next step's outputs and read its rows (after this step's
+    // updates of the same rows), when neither sentence ends here.
+    const bool cont = (p0 + 1 < len[0]) && (p1 + 1 < len[1]);
+    int osn = 0;
+    if (cont) {
+      const int np = plan ^ 1;
+      osn = make_outputs(np, s.tok[p0 + 1], s.tok[Lmax + p1 + 1], (neg_count + 1) % nbn);
+      load_output_l1(woh, s.plans + np * 2 * kOutRows, osn, g, m, o1);
+    }
+    gemm3(S, C, o3);
+    if (cont) load_output_l3(woh, s.plans + (plan ^ 1) * 2 * kOutRows, osn, g, m, o3);
+
+    steps++;
+    n_ctx += (u64)nin;
+    n_outctx += (u64)(os * nin);
+
+    // Transition: a finished sentence writes its live slots back; a
+    // continuing one slides its window by one position.
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int p = pos[q];
+      const int Lq = len[q];
+      int* const tq = s.tok + q * Lmax;
+      int* const sq = s.sop + q * Lmax;
+      pos[q] = p + 1;
+      if (p + 1 >= Lq) {
+        for (unsigned live = ~freem[q] & 0xFFFFFFu; live; live &= live - 1) {
+          const int sl = kSentSlots * q + __builtin_ctz(live);
+          float e[8];
+          slot_get(S, sl, e);
+          const int rk = __builtin_amdgcn_readfirstlane(s.srank[sl]);
+          if (g == ((sl >> 2) & 3))
+            wih.store((unsigned)rk * (D * 2) + 16 * m, pack8_sr(e, salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sl << 24)));
+        }
+        freem[q] = 0xFFFFFFu;
+        len[q] = 0;
+        done_sent++;
+        sentences++;
+        continue;
+      }
+      const int x = p - W, y = p + W + 1;
+      if (x >= 0 && __builtin_amdgcn_readfirstlane(s.nxt[q * Lmax + x]) > y) {
+        // x's rank leaves the window: write its slot back and free it.
+        const int sx = __builtin_amdgcn_readfirstlane(sq[x]);
+        float e[8];
+        slot_get(S, sx, e);
+        if (g == ((sx >> 2) & 3))
+          wih.store((unsigned)__builtin_amdgcn_readfirstlane(tq[x]) * (D * 2) + 16 * m,
+                    pack8_sr(e, salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sx << 24)));
+        freem[q] |= 1u << (sx - kSentSlots * q);
+      }
+      if (in_slot_src[q] >= 0) {
+        const int sy = __builtin_amdgcn_readfirstlane(sq[in_slot_src[q]]);
+        if (lane == 0) sq[y] = sy;
+      } else if (in_slot_src[q] == -1) {
+        const int k = __ffs((int)freem[q]) - 1;
+        freem[q] &= ~(1u << k);
+        const int sl = kSentSlots * q + k;
+        slot_put(S, sl, in_row[q], g);
+        if (lane == 0) { sq[y] = sl; s.srank[sl] = tq[y]; }
+      }
+    }
+    if (done_sent >= 2) {
+      refresh = 1;
+      done_sent = 0;
+    }
+    planned = cont;
+    if (cont) { plan ^= 1; pre_os = osn; }
+    salt = hash32(salt + 0x9E3779B9u);
+    wave_sync();
+  }
+
+  if (lane == 0) {
+    a.worker_steps[wk] = steps;
+    a.worker_alpha[wk] = alpha;
+    atomicAdd(&a.counters[kSteps], steps);
+    atomicAdd(&a.counters[kRawWords], raw);
+    atomicAdd(&a.counters[kSentences], sentences);
+    atomicAdd(&a.counters[kContexts], n_ctx);
+    atomicAdd(&a.counters[kOutCtx], n_outctx);
+  }
+}
+
+// One step of the window-slot form on given rows (test-facing): O (os x D
+// float), slot rows I (48 x D float, slots 0..23 first sentence), context
+// count per slot, output multiplicities.  Writes the coefficients (os x 48),
+// md (os x D) and the updated slot rows (48 x D), each through the same
+// device code as the trainer.
+__global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha,
+                                                  const float* exp_table, const float* O, const float* I,
+                                                  float* corr, float* md_out, float* I_out) {
+  __shared__ __align__(16) unsigned char lds[kSlots * RCS + kOutRows * CBS + 2 * kSlots * 4 + 4096 + 64];
+  const int lane = threadIdx.x, g = lane >> 4, m = lane & 15;
+  unsigned char* rc = lds;
+  unsigned char* cb = lds + kSlots * RCS;
+  int* ml = reinterpret_cast<int*>(cb + kOutRows * CBS);
+  int* plan = ml + kSlots;
+  float* expt = reinterpret_cast<float*>(plan + kSlots);
+  for (int x = lane; x < kExpTable; x += 64) expt[x] = exp_table[x];
+  for (int x = lane; x < (kOutRows * CBS) / 16; x += 64) reinterpret_cast<u32x4*>(cb)[x] = (u32x4){0u, 0u, 0u, 0u};
+  if (lane < kSlots) ml[lane] = mult[lane];
+  if (lane < kOutRows) { plan[lane] = lane; plan[kOutRows + lane] = lane < os ? meta[lane] : 0; }
+  Slots S;
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++)
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) S.R[rt][t][r] = I[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t];
+  // Output rows as bf16, in the two register layouts.
+  u32x4 o1[KB1], o3[4];
+  {
+    const int r1 = m < os ? m : 0;
+#pragma unroll
+    for (int ks = 0; ks < KB1; ks++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const float* src = O + (size_t)r1 * D + 32 * ks + 8 * g + 2 * k;
+        o1[ks][k] = pack_rne(src[0], src[1]);
+      }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int i = 4 * g + e;
+      const float* src = O + (size_t)(i < os ? i : 0) * D + 8 * m;
+#pragma unroll
+      for (int k = 0; k < 4; k++) o3[e][k] = pack_rne(src[2 * k], src[2 * k + 1]);
+    }
+  }
+  __syncthreads();
+  ring_copy(S, rc, g, m);
+  int meta4[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) meta4[r] = plan[kOutRows + 4 * g + r];
+  __syncthreads();
+  const StepLds L{rc, cb, ml};
+  float C[3][4];
+  gemm1_coef(o1, L, meta4, os, alpha, expt, g, m, C);
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (4 * g + r < os) corr[(size_t)(4 * g + r) * kSlots + 16 * rt + m] = C[rt][r];
+  __syncthreads();
+  f32x4 md[NT];
+  gemm2(S, L, g, m, md);
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    if (4 * g + r < os)
+#pragma unroll
+      for (int t = 0; t < NT; t++) md_out[(size_t)(4 * g + r) * D + 8 * m + t] = md[t][r];
+  gemm3(S, C, o3);
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++)
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) I_out[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t] = S.R[rt][t][r];
+}
+
+}  // namespace
+
+bool fast_supported(int D_, int K, int W, uint32_t V) {
+  return D_ == D && K + 2 <= kOutRows && 2 * W + 1 <= kSentSlots && W >= 1 &&
+         (uint64_t)V * D * 2 < (1ull << 31);
+}
+
+size_t fast_lds_bytes(int Lmax, int K) {
+  size_t off[6];
+  return fast_layout(Lmax, K, off);
+}
+
+size_t fast_block_lds_bytes(int Lmax, int K, int wpb) { return exp_bytes() + (size_t)wpb * fast_lds_bytes(Lmax, K); }
+
+int fast_workers_per_block(int Lmax, int K, size_t lds_per_block) {
+  const size_t per = fast_lds_bytes(Lmax, K);
+  if (lds_per_block < exp_bytes() + per) return 0;
+  const size_t n = (lds_per_block - exp_bytes()) / per;
+  return (int)(n < (size_t)kFastMaxWpb ? n : (size_t)kFastMaxWpb);
+}
+
+hipError_t prepare_train_fast(const TrainArgs& a) {
+  if (a.wpb < 1 || a.wpb > kFastMaxWpb) return hipErrorInvalidValue;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train_fast<kFastMaxWpb>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)fast_block_lds_bytes(a.Lmax, a.K, a.wpb));
+}
+
+hipError_t launch_train_fast(const TrainArgs& a, hipStream_t st) {
+  if (a.wpb < 1 || a.wpb > kFastMaxWpb) return hipErrorInvalidValue;
+  const size_t lds = fast_block_lds_bytes(a.Lmax, a.K, a.wpb);
+  hipLaunchKernelGGL((k_train_fast<kFastMaxWpb>), dim3((a.workers + a.wpb - 1) / a.wpb), dim3(64 * a.wpb), lds, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha, const float* exp_table,
+                            const float* O, const float* I, float* corr, float* md, float* I_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_fast, dim3(1), dim3(64), 0, st, os, meta, mult, alpha, exp_table, O, I, corr, md, I_out);
+  return hipGetLastError();
+}
+
+}  // namespace dger

@@ -152,6 +152,9 @@ struct dger_ctx {
   // FP8 form: master rows of Wih / Woh and the two matrix descriptors.
   DevBuf wm_d, om_d, desc_d;
   int n_master = 0;
+  // BF16-MFMA form on the window-slot trainer (sgns_fast.hip): set when the
+  // shape is one it supports, else the form runs on the buffered trainer.
+  bool fast = false;
 
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -363,6 +366,8 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   c->o = *opts;
   c->V = v_num;
   c->D = opts->size;
+  c->fast = opts->precision == DGER_PRECISION_BF16_MFMA &&
+            dger::fast_supported(opts->size, opts->negative, opts->window, v_num);
   auto bail = [&](int code) { delete c; return code; };
   if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess)
@@ -483,7 +488,7 @@ static int auto_workers(dger_ctx* c, size_t block_lds, int wpb) {
 }
 
 static int ensure_worker_buffers(dger_ctx* c, int workers) {
-  const size_t per = dger::worker_scratch_floats(c->Lmax, c->o.negative, c->D);
+  const size_t per = c->fast ? 0 : dger::worker_scratch_floats(c->Lmax, c->o.negative, c->D);
   if (c->scratch_d.ensure((size_t)workers * per * 4) || c->wsteps_d.ensure((size_t)workers * 8) ||
       c->walpha_d.ensure((size_t)workers * 4))
     return fail(DGER_E_ALLOC, "scratch allocation failed");
@@ -530,13 +535,21 @@ static dger::TrainArgs make_args(dger_ctx* c, int workers, int wpb, int32_t* rec
 // and the trainer kernel, timed with events on the context's stream.  With
 // use_graph the pair is captured once and the graph replayed while the
 // arguments stay the same.
+static hipError_t launch_pass_kernel(dger_ctx* c, const dger::TrainArgs& a) {
+  return c->fast ? dger::launch_train_fast(a, c->stream) : dger::launch_train(a, c->o.precision, c->stream);
+}
+
 static int run_pass(dger_ctx* c, const dger::TrainArgs& a, bool use_graph, float* ms) {
-  DGER_HIP(dger::prepare_train(a, c->o.precision));
+  if (c->fast) {
+    DGER_HIP(dger::prepare_train_fast(a));
+  } else {
+    DGER_HIP(dger::prepare_train(a, c->o.precision));
+  }
   if (use_graph && (!c->graph_exec || std::memcmp(&c->graph_args, &a, sizeof(a)) != 0)) {
     c->drop_graph();
     DGER_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = dger::launch_zero_u64(a.counters, dger::kNumCounters, c->stream);
-    if (e == hipSuccess) e = dger::launch_train(a, c->o.precision, c->stream);
+    if (e == hipSuccess) e = launch_pass_kernel(c, a);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (e == hipSuccess) e = e2;
@@ -554,7 +567,7 @@ static int run_pass(dger_ctx* c, const dger::TrainArgs& a, bool use_graph, float
     DGER_HIP(hipGraphLaunch(c->graph_exec, c->stream));
   } else {
     DGER_HIP(dger::launch_zero_u64(a.counters, dger::kNumCounters, c->stream));
-    DGER_HIP(dger::launch_train(a, c->o.precision, c->stream));
+    DGER_HIP(launch_pass_kernel(c, a));
   }
   DGER_HIP(hipEventRecord(c->ev1, c->stream));
   DGER_HIP(hipStreamSynchronize(c->stream));
@@ -609,13 +622,17 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
   DGER_HIP(hipGetDeviceProperties(&prop, c->o.device));
   // Workgroups may take more LDS than the default per-block figure by opting in.
   const size_t lds_block_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
-  int wpb = dger::train_workers_per_block(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision,
-                                          lds_block_max);
+  auto block_lds = [&](int n) {
+    return c->fast ? dger::fast_block_lds_bytes(c->Lmax, c->o.negative, n)
+                   : dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision, n);
+  };
+  int wpb = c->fast ? dger::fast_workers_per_block(c->Lmax, c->o.negative, lds_block_max)
+                    : dger::train_workers_per_block(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision,
+                                                    lds_block_max);
   if (wpb < 1)
     return fail(DGER_E_UNSUPPORTED,
                 "sentence length / window / negative need " +
-                    std::to_string(dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D,
-                                                               c->o.precision, 1)) +
+                    std::to_string(block_lds(1)) +
                     " bytes of LDS, more than the device allows");
   const int wpb_max = wpb;
   if (workers <= 0) workers = c->o.workers;
@@ -626,13 +643,12 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
     c->tuned_workers = workers;
   }
   if (workers <= 0)
-    workers = auto_workers(
-        c, dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision, wpb), wpb);
+    workers = auto_workers(c, block_lds(wpb), wpb);
   wpb = std::min(wpb, workers);
   if (c->o.debug > 2)
     fprintf(stderr, "dger: %d workers, %d per workgroup, LDS %zu bytes per workgroup (limit %zu, opt-in %zu)\n",
             workers, wpb,
-            dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision, wpb),
+            block_lds(wpb),
             (size_t)prop.sharedMemPerBlock, (size_t)prop.sharedMemPerBlockOptin);
   const int stride = dger::record_stride(c->o.window, c->o.negative);
   if (const int rc = ensure_worker_buffers(c, workers); rc != DGER_OK) return rc;
@@ -861,6 +877,32 @@ int dger_sgns_step(int32_t D, int32_t n_out, int32_t n_ctx, int32_t n_ctx_first,
   DGER_HIP(hipMemcpy(corr, dc.p, (size_t)n_out * n_ctx * 4, hipMemcpyDeviceToHost));
   DGER_HIP(hipMemcpy(md, dmd.p, (size_t)n_out * D * 4, hipMemcpyDeviceToHost));
   DGER_HIP(hipMemcpy(gin, dg.p, (size_t)n_ctx * D * 4, hipMemcpyDeviceToHost));
+  return DGER_OK;
+}
+
+int dger_sgns_step_fast(int32_t n_out, const int32_t* meta, const int32_t* mult, float alpha, const float* O,
+                        const float* I, float* corr, float* md, float* I_out) {
+  const int D = dger::kFastDim, S = 48;
+  if (n_out < 2 || n_out > 16 || !meta || !mult || !O || !I || !corr || !md || !I_out)
+    return fail(DGER_E_ARG, "bad window-slot step arguments");
+  float exp_table[dger::kExpTable];
+  build_exp_table(exp_table);
+  DevBuf dm, dmu, de, dO, dI, dc, dmd, dI2;
+  if (dm.ensure(n_out * 4) || dmu.ensure(S * 4) || de.ensure(sizeof(exp_table)) || dO.ensure((size_t)n_out * D * 4) ||
+      dI.ensure((size_t)S * D * 4) || dc.ensure((size_t)n_out * S * 4) || dmd.ensure((size_t)n_out * D * 4) ||
+      dI2.ensure((size_t)S * D * 4))
+    return fail(DGER_E_ALLOC, "device allocation failed");
+  DGER_HIP(hipMemcpy(dm.p, meta, n_out * 4, hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(dmu.p, mult, S * 4, hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(de.p, exp_table, sizeof(exp_table), hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(dO.p, O, (size_t)n_out * D * 4, hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(dI.p, I, (size_t)S * D * 4, hipMemcpyHostToDevice));
+  DGER_HIP(dger::launch_step_fast(n_out, dm.as<int32_t>(), dmu.as<int32_t>(), alpha, de.as<float>(), dO.as<float>(),
+                                  dI.as<float>(), dc.as<float>(), dmd.as<float>(), dI2.as<float>(), nullptr));
+  DGER_HIP(hipDeviceSynchronize());
+  DGER_HIP(hipMemcpy(corr, dc.p, (size_t)n_out * S * 4, hipMemcpyDeviceToHost));
+  DGER_HIP(hipMemcpy(md, dmd.p, (size_t)n_out * D * 4, hipMemcpyDeviceToHost));
+  DGER_HIP(hipMemcpy(I_out, dI2.p, (size_t)S * D * 4, hipMemcpyDeviceToHost));
   return DGER_OK;
 }
 

@@ -78,6 +78,20 @@ int train_workers_per_block(int Lmax, int K, int W, int D, int precision, size_t
 size_t train_block_lds_bytes(int Lmax, int K, int W, int D, int precision, int wpb);
 DGER_HD inline int record_stride(int W, int K) { return 8 + 4 * W + 2 * (K + 2); }
 
+// Window-slot trainer of the BF16-MFMA form (sgns_fast.hip): bf16 rows of
+// length kFastDim, window slots in registers, contractions on MFMA.
+constexpr int kFastDim = 128;
+constexpr int kFastMaxWpb = 4;  // workers (wavefronts) per workgroup: one per SIMD
+bool fast_supported(int D, int K, int W, uint32_t V);
+size_t fast_lds_bytes(int Lmax, int K);
+size_t fast_block_lds_bytes(int Lmax, int K, int wpb);
+int fast_workers_per_block(int Lmax, int K, size_t lds_per_block);
+hipError_t prepare_train_fast(const TrainArgs& a);
+hipError_t launch_train_fast(const TrainArgs& a, hipStream_t s);
+// One step of the window-slot form on host-staged rows (test-facing).
+hipError_t launch_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha, const float* exp_table,
+                            const float* O, const float* I, float* corr, float* md, float* I_out, hipStream_t s);
+
 hipError_t launch_init_net(float* wih, uint64_t n, int D, hipStream_t s);
 hipError_t launch_table_fill(int32_t* table, uint64_t table_size, const int64_t* starts, int n_starts,
                              hipStream_t s);

@@ -109,6 +109,7 @@ _SIGS = {
     "dger_ctx_save": (C.c_int, [_vp, C.c_char_p, _i32]),
     "dger_ctx_destroy": (None, [_vp]),
     "dger_sgns_step": (C.c_int, [_i32, _i32, _i32, _i32, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp]),
+    "dger_sgns_step_fast": (C.c_int, [_i32, _vp, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp]),
     "dger_loss_fwd_f32": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp]),
     "dger_loss_bwd_f32": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, C.c_float, _vp, _vp]),
     "dger_loss_fwd_f64": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp]),
@@ -417,6 +418,30 @@ def step(O, I, fin: int, meta, alpha: float):
     _check(lib().dger_sgns_step(D, os_, nin, fin, _ptr(meta), alpha, _ptr(O), _ptr(I), _ptr(corr),
                                 _ptr(md), _ptr(gin)))
     return corr, md, gin
+
+
+FAST_DIM = 128
+FAST_SLOTS = 48
+
+
+def step_fast(O, I, mult, meta, alpha: float):
+    """One step of the window-slot BF16-MFMA trainer on the device
+    (dger_sgns_step_fast): O (n_out x 128), I (48 x 128 slot rows), mult (48
+    contexts per slot), meta (n_out).  Returns corr (n_out x 48), md
+    (n_out x 128) and the updated slot rows (48 x 128)."""
+    O = np.ascontiguousarray(O, np.float32)
+    I = np.ascontiguousarray(I, np.float32)
+    mult = _i32arr(mult)
+    meta = _i32arr(meta)
+    os_ = O.shape[0]
+    if O.shape[1] != FAST_DIM or I.shape != (FAST_SLOTS, FAST_DIM) or mult.size != FAST_SLOTS or meta.size != os_:
+        raise ValueError("shapes: O (n_out, 128), I (48, 128), mult (48,), meta (n_out,)")
+    corr = np.empty((os_, FAST_SLOTS), np.float32)
+    md = np.empty((os_, FAST_DIM), np.float32)
+    I2 = np.empty((FAST_SLOTS, FAST_DIM), np.float32)
+    _check(lib().dger_sgns_step_fast(os_, _ptr(meta), _ptr(mult), alpha, _ptr(O), _ptr(I), _ptr(corr), _ptr(md),
+                                     _ptr(I2)))
+    return corr, md, I2
 
 
 class DeviceArray:

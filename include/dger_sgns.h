@@ -40,7 +40,13 @@ extern "C" {
 
 #define DGER_PRECISION_FP32 0   /* FP32 rows, FP32 FMA arithmetic */
 #define DGER_PRECISION_BF16 1   /* BF16 rows in HBM, FP32 working buffers, FP32 FMA contractions */
-#define DGER_PRECISION_BF16_MFMA 2 /* BF16 rows, contractions on MFMA (bf16 operands, FP32 accumulate) */
+#define DGER_PRECISION_BF16_MFMA 2 /* BF16 rows, contractions on MFMA (bf16 operands, FP32 accumulate).
+                                   At size 128, window <= 11, negative <= 14 it runs the window-slot
+                                   trainer: window rows in registers, read from Wih on entering the
+                                   window and written back (stochastic rounding) on leaving it,
+                                   target and negative rows updated in Woh directly, all three
+                                   contractions on MFMA; the index stream is the reference's.  Other
+                                   shapes keep the reference's buffers with GEMM1 on MFMA. */
 #define DGER_PRECISION_FP8 3    /* FP8 e4m3 rows with a per-row FP32 scale, FP32 master rows for the
                                    top 1% ranks by count, stochastic rounding on every row store;
                                    contractions as BF16_MFMA.  Needs size % 4 == 0, window <= 16,
@@ -177,6 +183,21 @@ int32_t dger_record_stride(int32_t window, int32_t negative);
 int dger_sgns_step(int32_t D, int32_t n_out, int32_t n_ctx, int32_t n_ctx_first,
                    const int32_t *meta, float alpha, const float *O, const float *I,
                    float *corr, float *md, float *gin);
+
+/* One step of the BF16-MFMA form's window-slot trainer on host-staged rows
+ * (the same device code as that trainer; see DGER_PRECISION_BF16_MFMA).
+ * Row length 128.  O (n_out x 128, n_out in [2, 16]) holds the step's output
+ * rows, meta (n_out) their multiplicities; I (48 x 128) holds the 48 window
+ * slots: slots 0..23 belong to the first sentence (their contexts label
+ * output 0 positive), 24..47 to the second (output n_out-1 positive); mult
+ * (48) is the number of the step's contexts in each slot (0 = none).  The
+ * contractions take bf16 operands and accumulate in FP32.  Outputs: corr
+ * (n_out x 48) the coefficients mult * meta * (label - sigmoid) * alpha, md
+ * (n_out x 128) the output-row updates, I_out (48 x 128) the slot rows after
+ * the update.  Each step of dsgl.hpp:1241-1333 with repeated contexts merged
+ * into one slot. */
+int dger_sgns_step_fast(int32_t n_out, const int32_t *meta, const int32_t *mult, float alpha,
+                        const float *O, const float *I, float *corr, float *md, float *I_out);
 
 /* SGNS logistic loss and its adjoint over minibatches of shared negatives,
  * on DEVICE pointers.  Minibatch m pairs contexts ctx[m*n_ctx .. +n_ctx) (rows

@@ -51,7 +51,7 @@ class OrcOpts(C.Structure):
         ("dim", C.c_int32), ("window", C.c_int32), ("negative", C.c_int32),
         ("iter", C.c_int32), ("threads", C.c_int32), ("workers", C.c_int32),
         ("sample", C.c_float), ("alpha", C.c_float), ("table_size", C.c_int64),
-        ("init_model", C.c_int32), ("parallel", C.c_int32),
+        ("init_model", C.c_int32), ("parallel", C.c_int32), ("semantics", C.c_int32),
     ]
 
 
@@ -171,10 +171,15 @@ def record_stride(window: int, negative: int) -> int:
     return int(lib().orc_record_stride(window, negative))
 
 
+SEMANTICS = {"reference": 0, "slots": 1}
+
+
 def make_opts(dim=128, window=10, negative=5, iter=1, threads=2, workers=1, sample=1e-3,
-              alpha=0.01, table_size=TABLE_SIZE, init_model=1, parallel=0) -> OrcOpts:
+              alpha=0.01, table_size=TABLE_SIZE, init_model=1, parallel=0, semantics="reference") -> OrcOpts:
+    """semantics: "reference" (dsgl.hpp's buffers) or "slots" (the update
+    rules of the library's BF16-MFMA window-slot trainer)."""
     return OrcOpts(dim, window, negative, iter, threads, workers, sample, alpha, table_size,
-                   init_model, parallel)
+                   init_model, parallel, SEMANTICS[semantics])
 
 
 def train(corpus: np.ndarray, cn_by_id: np.ndarray, opts: OrcOpts, wih=None, woh=None,

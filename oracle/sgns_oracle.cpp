@@ -245,6 +245,8 @@ typedef struct orc_opts {
   int64_t table_size;   // 1e8 in the reference
   int32_t init_model;   // 1: InitNet + zero Woh; 0: use the given matrices
   int32_t parallel;     // 0: workers run one after another; 1: OpenMP Hogwild
+  int32_t semantics;    // 0: the reference's buffers; 1: window slots (the BF16-MFMA
+                        //    window-slot trainer's update rules, see run_worker_slots)
 } orc_opts;
 
 typedef struct orc_stats {
@@ -509,6 +511,247 @@ WorkerResult run_worker(Shared& sh, int w, int32_t* rec, int64_t rec_cap) {
   return res;
 }
 
+// One compute thread with the window-slot update rules of the library's
+// BF16-MFMA trainer (gdistger-re_amd/csrc/sgns_fast.hip), in FP32.  The index
+// stream is the reference's (same loads, draws, windows and output lists as
+// run_worker, dsgl.hpp:786-1160); the updates differ from dsgl.hpp:1314-1407:
+//   * a rank's working row exists while one of its positions is inside the
+//     window [p-W, p+W]; it is read from Wih when the rank enters (rows of
+//     the first window at the sentence load, an entering row at the start of
+//     the step whose transition brings it in) and written back to Wih when
+//     its last in-window position leaves, or at the sentence end;
+//   * target and negative rows are read from Woh at the step and updated
+//     there directly (no negative buffer);
+//   * repeated contexts share their row: one coefficient times their count.
+WorkerResult run_worker_slots(Shared& sh, int w, int32_t* rec, int64_t rec_cap) {
+  const orc_opts& o = *sh.o;
+  const Tables& t = *sh.t;
+  const int D = o.dim, W = o.window, K = o.negative;
+  const int nparts = o.workers;
+  const u64 N = sh.n_tokens;
+  const int stride = orc_record_stride(W, K);
+  WorkerResult res;
+  u64 per = N / (u64)nparts;
+  u64 start = per * (u64)w;
+  u64 count = per;
+  if (N % (u64)nparts > 0 && w == nparts - 1) count = N - (u64)(nparts - 1) * per;
+  const u64 end = start + count;
+  u64 rng = (u64)w;
+  const float a0 = o.alpha;
+  float alpha = a0;
+  int iters_left = o.iter;
+  u64 wc = 0, last_wc = 0, cur = start;
+
+  struct Win {
+    int len = 0, pos = 0;
+    std::vector<int> tok;
+    std::vector<int> rank;    // ranks with a row in the window
+    std::vector<float> row;   // their rows (rank.size() x D)
+    int find(int r) const {
+      for (size_t i = 0; i < rank.size(); i++) if (rank[i] == r) return (int)i;
+      return -1;
+    }
+  };
+  Win s[2];
+  for (auto& x : s) x.tok.assign(kMaxSentence, 0);
+  int done_sentences = 0;
+  bool refresh = true;
+  int neg_count = 0, nbn = 1;
+  std::vector<int> neg_words;
+  std::vector<int> oidx(K + 2), ometa(K + 2);
+  std::vector<float> O((size_t)(K + 2) * D), md((size_t)(K + 2) * D);
+
+  auto in_window_before = [&](const Win& x, int y, int lo) {  // rank of y at a position in [lo, y)
+    for (int c = std::max(lo, 0); c < y; c++) if (x.tok[c] == x.tok[y]) return true;
+    return false;
+  };
+  auto write_back = [&](Win& x, int e) {
+    std::memcpy(sh.wih + (size_t)x.rank[e] * D, &x.row[(size_t)e * D], sizeof(float) * D);
+    x.rank.erase(x.rank.begin() + e);
+    x.row.erase(x.row.begin() + (size_t)e * D, x.row.begin() + (size_t)(e + 1) * D);
+  };
+  auto add_row = [&](Win& x, int r, const float* src) {
+    x.rank.push_back(r);
+    x.row.insert(x.row.end(), src, src + D);
+  };
+  auto load = [&](Win& x) {
+    while (x.len == 0) {
+      if (cur >= end || cur >= N) break;
+      for (;;) {
+        int id = sh.corpus[cur];
+        wc++;
+        cur++;
+        if (id == -1 || cur >= end) break;
+        int r = t.id_rank[id];
+        if (o.sample > 0) {
+          rng = lcg_next(rng);
+          if (t.keep_thr[r] < (float)(rng & 0xFFFF) / 65536.f) continue;
+        }
+        x.tok[x.len++] = r;
+        if (x.len >= kMaxSentence) break;
+      }
+      x.pos = 0;
+      if (x.len == 1 || x.len == 0) { x.len = 0; continue; }
+      x.rank.clear();
+      x.row.clear();
+      for (int c = 0; c <= std::min(W, x.len - 1); c++)
+        if (x.find(x.tok[c]) < 0) add_row(x, x.tok[c], sh.wih + (size_t)x.tok[c] * D);
+    }
+  };
+
+  std::vector<float> in_row[2] = {std::vector<float>(D), std::vector<float>(D)};
+  for (;;) {
+    if (wc - last_wc > 10000) {
+      u64 diff = wc - last_wc;
+      u64 total = sh.word_count_actual.fetch_add(diff) + diff;
+      last_wc = wc;
+      alpha = a0 * (1 - (float)(total * 1ULL) / (float)((u64)o.iter * t.train_words + 1));
+      if (alpha < a0 * 0.0001f) alpha = a0 * 0.0001f;
+    }
+    load(s[0]);
+    load(s[1]);
+    if (cur >= end || cur >= N) {
+      sh.word_count_actual.fetch_add(wc - last_wc);
+      res.raw += wc;
+      if (--iters_left == 0) break;
+      wc = 0; cur = start; last_wc = 0;
+      s[0].len = 0; s[1].len = 0;
+      continue;
+    }
+    if (refresh) {
+      neg_words.clear();
+      nbn = std::max(s[0].len, s[1].len);
+      for (int i = 0; i < nbn; i++)
+        for (int k = 0; k < K; k++) {
+          rng = lcg_next(rng);
+          int v = t.table[(rng >> 16) % (u64)t.table_size];
+          if (!v) v = (int)(rng % (u64)(t.V - 1) + 1);
+          neg_words.push_back(v);
+        }
+      refresh = false;
+    }
+    const int t1 = s[0].tok[s[0].pos];
+    const int t2 = s[1].tok[s[1].pos];
+    rng = lcg_next(rng);
+    const int b = (int)(rng % (u64)W);
+    // contexts of both windows, as distinct rows with counts
+    std::vector<int> ctx_rank, ctx_q, ctx_e, ctx_mult;
+    int nin = 0, fin = 0;
+    std::vector<int> inputs;
+    for (int q = 0; q < 2; q++) {
+      const Win& x = s[q];
+      for (int i = b; i < 2 * W + 1 - b; i++) {
+        if (i == W) continue;
+        int c = x.pos - W + i;
+        if (c < 0) continue;
+        if (c >= x.len) break;
+        inputs.push_back(x.tok[c]);
+        nin++;
+        const int e = x.find(x.tok[c]);
+        size_t u = 0;
+        while (u < ctx_e.size() && !(ctx_q[u] == q && ctx_e[u] == e)) u++;
+        if (u == ctx_e.size()) { ctx_q.push_back(q); ctx_e.push_back(e); ctx_mult.push_back(0); }
+        ctx_mult[u]++;
+      }
+      if (q == 0) fin = nin;
+    }
+    neg_count++;
+    const int grp = neg_count % nbn;
+    int os = 1;
+    oidx[0] = t1;
+    ometa[0] = 1;
+    for (int k = 0; k < K; k++) {
+      int v = neg_words[(size_t)grp * K + k];
+      int hit = -1;
+      for (int i = 0; i < os; i++) if (oidx[i] == v) { hit = i; break; }
+      if (hit < 0) { oidx[os] = v; ometa[os] = 1; os++; }
+      else ometa[hit]++;
+    }
+    ometa[0] = 1;
+    oidx[os] = t2;
+    ometa[os] = 1;
+    os++;
+    // rows entering the windows at this step's transition, read now
+    bool enters[2] = {false, false};
+    for (int q = 0; q < 2; q++) {
+      const Win& x = s[q];
+      const int p = x.pos, y = p + W + 1;
+      if (p + 1 < x.len && y < x.len && !in_window_before(x, y, p - W)) {
+        enters[q] = true;
+        std::memcpy(in_row[q].data(), sh.wih + (size_t)x.tok[y] * D, sizeof(float) * D);
+      }
+    }
+    for (int i = 0; i < os; i++) std::memcpy(&O[(size_t)i * D], sh.woh + (size_t)oidx[i] * D, sizeof(float) * D);
+    if (rec && (int64_t)res.steps < rec_cap) {
+      int32_t* r = rec + (size_t)res.steps * stride;
+      for (int z = 0; z < stride; z++) r[z] = -1;
+      r[0] = t1; r[1] = t2; r[2] = b; r[3] = fin; r[4] = nin; r[5] = os;
+      std::memcpy(&r[6], &alpha, 4);
+      r[7] = grp;
+      for (int j = 0; j < nin; j++) r[8 + j] = inputs[j];
+      for (int i = 0; i < os; i++) { r[8 + 4 * W + i] = oidx[i]; r[8 + 4 * W + K + 2 + i] = ometa[i]; }
+    }
+    // coefficients over the distinct context rows (count times the table
+    // gradient), GEMM2 on the rows before the update, then GEMM3
+    const size_t nu = ctx_e.size();
+    std::vector<float> C(nu * (size_t)os);
+    for (size_t u = 0; u < nu; u++) {
+      const float* x = &s[ctx_q[u]].row[(size_t)ctx_e[u] * D];
+      for (int i = 0; i < os; i++) {
+        float f = 0.f;
+        for (int k = 0; k < D; k++) f = ORC_FMA(O[(size_t)i * D + k], x[k], f);
+        const int label = ctx_q[u] == 0 ? (i == 0) : (i == os - 1);
+        C[u * os + i] = sig_grad<float>(f, label, alpha, t.exp_table.data(), nullptr) * (float)(ometa[i] * ctx_mult[u]);
+      }
+    }
+    for (int i = 0; i < os; i++)
+      for (int d = 0; d < D; d++) {
+        float acc = 0.f;
+        for (size_t u = 0; u < nu; u++) acc = ORC_FMA(C[u * os + i], s[ctx_q[u]].row[(size_t)ctx_e[u] * D + d], acc);
+        md[(size_t)i * D + d] = acc;
+      }
+    for (size_t u = 0; u < nu; u++) {
+      float* x = &s[ctx_q[u]].row[(size_t)ctx_e[u] * D];
+      for (int d = 0; d < D; d++) {
+        float acc = 0.f;
+        for (int i = 0; i < os; i++) acc = ORC_FMA(C[u * os + i], O[(size_t)i * D + d], acc);
+        x[d] += acc;
+      }
+    }
+    for (int i = 0; i < os; i++) {
+      float* dst = sh.woh + (size_t)oidx[i] * D;
+      for (int d = 0; d < D; d++) dst[d] += md[(size_t)i * D + d];
+    }
+    res.steps++;
+    // transition
+    for (int q = 0; q < 2; q++) {
+      Win& x = s[q];
+      const int p = x.pos;
+      x.pos = p + 1;
+      if (p + 1 >= x.len) {
+        while (!x.rank.empty()) write_back(x, 0);
+        x.len = 0;
+        done_sentences++;
+        res.sentences++;
+        continue;
+      }
+      const int xo = p - W, y = p + W + 1;
+      if (xo >= 0) {
+        bool stays = false;
+        for (int c = xo + 1; c <= std::min(y, x.len - 1); c++) if (x.tok[c] == x.tok[xo]) stays = true;
+        if (!stays) write_back(x, x.find(x.tok[xo]));
+      }
+      if (enters[q]) add_row(x, x.tok[y], in_row[q].data());
+    }
+    if (done_sentences >= 2) {
+      refresh = true;
+      done_sentences = 0;
+    }
+  }
+  res.alpha = alpha;
+  return res;
+}
+
 int prepare_tables(const orc_opts* o, const int32_t* cn_by_id, int32_t V, Tables* t) {
   if (V < 2 || o->dim <= 0 || o->window <= 0 || o->negative < 0) return -1;
   t->V = V;
@@ -564,10 +807,12 @@ int orc_train(const orc_opts* o, const int32_t* corpus, uint64_t n_tokens, const
 #pragma omp parallel for schedule(static, 1) num_threads(o->workers)
 #endif
     for (int w = 0; w < o->workers; w++)
-      res[w] = run_worker(sh, w, rec ? rec + (size_t)w * rec_cap * stride : nullptr, rec_cap);
+      res[w] = (o->semantics ? run_worker_slots : run_worker)(sh, w, rec ? rec + (size_t)w * rec_cap * stride : nullptr,
+                                                             rec_cap);
   } else {
     for (int w = 0; w < o->workers; w++)
-      res[w] = run_worker(sh, w, rec ? rec + (size_t)w * rec_cap * stride : nullptr, rec_cap);
+      res[w] = (o->semantics ? run_worker_slots : run_worker)(sh, w, rec ? rec + (size_t)w * rec_cap * stride : nullptr,
+                                                             rec_cap);
   }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (stats) {

@@ -14,19 +14,20 @@ from corpora import random_corpus
 pytestmark = pytest.mark.gpu
 
 
-def opts(graph, autotune=0, precision="fp32"):
-    return G.default_opts(size=64, window=5, negative=5, sample=1e-3, alpha=0.025, iter=1, threads=2,
+def opts(graph, autotune=0, precision="fp32", D=64):
+    return G.default_opts(size=D, window=5, negative=5, sample=1e-3, alpha=0.025, iter=1, threads=2,
                           table_size=1_000_000, precision=precision, debug=0, graph=graph, autotune=autotune)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16-mfma"])
-def test_graph_replays_match_eager(precision):
+# bf16-mfma at size 128 runs the window-slot trainer, at 64 the buffered one
+@pytest.mark.parametrize("precision,D", [("fp32", 64), ("bf16-mfma", 64), ("bf16-mfma", 128)])
+def test_graph_replays_match_eager(precision, D):
     corpus, cn = random_corpus(7, V=2000, n_walks=800, lmin=2, lmax=70)
-    print(f"graph replay {precision} tokens={corpus.size}", flush=True)
-    with G.Trainer(opts(0, precision=precision), cn, corpus) as tr:
+    print(f"graph replay {precision} D={D} tokens={corpus.size}", flush=True)
+    with G.Trainer(opts(0, precision=precision, D=D), cn, corpus) as tr:
         est, _, _ = tr.train(workers=1)
         ewih, ewoh = tr.get_model()
-    with G.Trainer(opts(1, precision=precision), cn, corpus) as tr:
+    with G.Trainer(opts(1, precision=precision, D=D), cn, corpus) as tr:
         for rep in range(3):
             tr.reset_model()
             gst, _, _ = tr.train(workers=1)

@@ -87,11 +87,13 @@ def test_multi_worker_index_streams_bit_exact():
         assert np.array_equal(grec[w, :n][:, idx], rec[w, :n][:, idx]), w
 
 
-@pytest.mark.parametrize("precision,bound_in,bound_out", [("bf16", 0.04, 0.10), ("bf16-mfma", 0.15, 0.25)])
-def test_bf16_forms_same_index_stream_close_model(precision, bound_in, bound_out):
+# bf16-mfma at size 64 is outside the window-slot trainer's shapes (size 128)
+# and runs on the buffered trainer with GEMM1 on MFMA.
+@pytest.mark.parametrize("precision,D,bound_in,bound_out", [("bf16", 128, 0.04, 0.10), ("bf16-mfma", 64, 0.15, 0.25)])
+def test_bf16_forms_same_index_stream_close_model(precision, D, bound_in, bound_out):
     corpus, cn = random_corpus(9, n_walks=800)
-    wih, woh, st, rec, ws = O.train(corpus, cn, orc_opts(128, 5, 5, 1e-3), rec_cap=30000)
-    with G.Trainer(gpu_opts(128, 5, 5, 1e-3, precision=precision), cn, corpus) as tr:
+    wih, woh, st, rec, ws = O.train(corpus, cn, orc_opts(D, 5, 5, 1e-3), rec_cap=30000)
+    with G.Trainer(gpu_opts(D, 5, 5, 1e-3, precision=precision), cn, corpus) as tr:
         gst, grec, _ = tr.train(workers=1, rec_cap=30000)
         gwih, gwoh = tr.get_model()
     n = int(st.steps)
