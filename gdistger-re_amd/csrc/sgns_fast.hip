@@ -245,19 +245,20 @@ __device__ __forceinline__ void ring_copy(const Slots& S, unsigned char* rc, int
     }
 }
 
-// GEMM1 + sigmoid + coefficients.  o1[ks]: row m of the outputs, elements
-// 32 ks + 8 g .. +8.  On return C[rt][r] = coefficient of slot 16 rt + m and
+// GEMM1 + sigmoid + coefficients.  ost: the step's output rows in bf16,
+// RCS bytes apart (GEMM1's A operand is row m, elements 32 ks + 8 g .. +8).
+// On return C[rt][r] = coefficient of slot 16 rt + m and
 // output 4 g + r (0 unless the slot is a context and the output exists), cb
 // holds the coefficients for GEMM2.  meta4[r] = multiplicity of output 4g+r
 // (0 past the step's outputs); os = number of outputs.
-__device__ __forceinline__ void gemm1_coef(const u32x4 (&o1)[KB1], const StepLds& L, const int (&meta4)[4], int os,
+__device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepLds& L, const int (&meta4)[4], int os,
                                            float alpha, const float* expt, int g, int m, float (&C)[3][4]) {
   f32x4 f[3];
 #pragma unroll
   for (int rt = 0; rt < 3; rt++) f[rt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < KB1; ks++) {
-    const bf16x8 a = __builtin_bit_cast(bf16x8, o1[ks]);
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(ost + m * RCS + (32 * ks + 8 * g) * 2);
 #pragma unroll
     for (int rt = 0; rt < 3; rt++) {
       const bf16x8 b = *reinterpret_cast<const bf16x8*>(L.rc + (16 * rt + m) * RCS + (32 * ks + 8 * g) * 2);
@@ -273,18 +274,22 @@ __device__ __forceinline__ void gemm1_coef(const u32x4 (&o1)[KB1], const StepLds
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int i = 4 * g + r;
-      const int label = second ? (i == os - 1 ? 1 : 0) : (i == 0 ? 1 : 0);
-      const int cnt = meta4[r] * mu;
-      const float gr = sig_grad(f[rt][r], label, alpha, expt);
-      C[rt][r] = cnt ? gr * (float)cnt : 0.f;
+      const float lab = (second ? (i == os - 1) : (i == 0)) ? 1.f : 0.f;
+      // (label - sigmoid) * alpha * count, branch-free: a count of 0 (no
+      // context in the slot, or an output past os) gives exactly 0
+      const float x = f[rt][r];
+      int idx = (int)((x + 6.0f) * 83.0f);
+      idx = min(max(idx, 0), kExpTable - 1);
+      const float e = (x > 6.0f) ? 1.f : ((x < -6.0f) ? 0.f : expt[idx]);
+      C[rt][r] = (lab - e) * (alpha * (float)(meta4[r] * mu));
       *reinterpret_cast<__bf16*>(L.cb + i * CBS + (m >> 2) * 32 + (4 * rt + (m & 3)) * 2) = (__bf16)C[rt][r];
     }
   }
 }
 
-// GEMM2: md[4g + r][8m + t] = acc[t][r], from the coefficients in cb and the
-// slots' values before this step's update.
-__device__ __forceinline__ void gemm2(const Slots& S, const StepLds& L, int g, int m, f32x4 (&acc)[NT]) {
+// GEMM2: md[t][r] = the update of output row 4g + r, element 8m + t, from
+// the coefficients in cb and the slots' values before this step's update.
+__device__ __forceinline__ void gemm2(const Slots& S, const StepLds& L, int g, int m, f32x4 (&md)[NT]) {
   const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(L.cb + m * CBS + g * 32);
   const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(L.cb + m * CBS + g * 32 + 16);
 #pragma unroll
@@ -300,8 +305,7 @@ __device__ __forceinline__ void gemm2(const Slots& S, const StepLds& L, int g, i
     b1[3] = 0u;
     f32x4 c = (f32x4){0.f, 0.f, 0.f, 0.f};
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, __builtin_bit_cast(bf16x8, b0), c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, __builtin_bit_cast(bf16x8, b1), c, 0, 0, 0);
-    acc[t] = c;
+    md[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, __builtin_bit_cast(bf16x8, b1), c, 0, 0, 0);
   }
 }
 
@@ -329,6 +333,7 @@ __device__ __forceinline__ void gemm3(Slots& S, const float (&C)[3][4], const u3
 // LDS carve-up of one worker of the window-slot trainer (16-byte aligned).
 struct FastLds {
   unsigned char* rc;  // kSlots x RCS
+  unsigned char* ost; // kOutRows x RCS: the step's output rows (bf16)
   unsigned char* cb;  // kOutRows x CBS
   int* mult;          // kSlots
   int* srank;         // kSlots: rank held by each slot
@@ -344,7 +349,7 @@ __host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15
 
 __host__ __device__ inline size_t fast_layout(int Lmax, int K, size_t* off) {
   size_t o = 0;
-  off[0] = o; o = al16(o + (size_t)kSlots * RCS);
+  off[0] = o; o = al16(o + (size_t)(kSlots + kOutRows) * RCS);
   off[1] = o; o = al16(o + (size_t)kOutRows * CBS);
   off[2] = o; o = al16(o + 2 * (size_t)kSlots * 4);
   off[3] = o; o = al16(o + 2 * 2 * (size_t)kOutRows * 4);
@@ -358,6 +363,7 @@ __device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K) {
   fast_layout(Lmax, K, off);
   FastLds s;
   s.rc = base + off[0];
+  s.ost = base + off[0] + (size_t)kSlots * RCS;
   s.cb = base + off[1];
   s.mult = reinterpret_cast<int*>(base + off[2]);
   s.srank = s.mult + kSlots;
@@ -373,27 +379,9 @@ __device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K) {
 
 __host__ __device__ inline size_t exp_bytes() { return al16((size_t)kExpTable * 4); }
 
-// Output rows of plan pl into the two register layouts (see the file comment);
-// rows past os read row 0 of the plan (their results are not used).
-__device__ __forceinline__ void load_outputs(const Mat& woh, const int* plan, int os, int g, int m, u32x4 (&o1)[KB1],
-                                             u32x4 (&o3)[4]) {
-  const int r1 = plan[m < os ? m : 0];
-#pragma unroll
-  for (int ks = 0; ks < KB1; ks++) o1[ks] = woh.load((unsigned)r1 * (D * 2) + (32 * ks + 8 * g) * 2);
-#pragma unroll
-  for (int e = 0; e < 4; e++) {
-    const int i = 4 * g + e;
-    const int r3 = plan[i < os ? i : 0];
-    o3[e] = woh.load((unsigned)r3 * (D * 2) + 16 * m);
-  }
-}
-
-__device__ __forceinline__ void load_output_l1(const Mat& woh, const int* plan, int os, int g, int m, u32x4 (&o1)[KB1]) {
-  const int r1 = plan[m < os ? m : 0];
-#pragma unroll
-  for (int ks = 0; ks < KB1; ks++) o1[ks] = woh.load((unsigned)r1 * (D * 2) + (32 * ks + 8 * g) * 2);
-}
-
+// Output rows of a plan for GEMM3's B operand (rows 4g .. 4g+3, elements
+// 8m .. 8m+7); rows past os read row 0 of the plan (their results are not
+// used).
 __device__ __forceinline__ void load_output_l3(const Mat& woh, const int* plan, int os, int g, int m, u32x4 (&o3)[4]) {
 #pragma unroll
   for (int e = 0; e < 4; e++) {
@@ -403,30 +391,14 @@ __device__ __forceinline__ void load_output_l3(const Mat& woh, const int* plan, 
   }
 }
 
-// Woh rows of the outputs += md (bf16 atomic adds of element pairs).
-__device__ __forceinline__ void scatter_outputs(void* woh, const int* plan, int os, int g, int m, const f32x4 (&md)[NT]) {
-  typedef short __attribute__((ext_vector_type(2))) short2v;
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int i = 4 * g + r;
-    if (i < os) {
-      uint16_t* row = static_cast<uint16_t*>(woh) + (size_t)plan[i] * D + 8 * m;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const unsigned w = pack_rne(md[2 * k][r], md[2 * k + 1][r]);
-        (void)__builtin_amdgcn_flat_atomic_fadd_v2bf16(reinterpret_cast<short2v*>(row) + k,
-                                                       __builtin_bit_cast(short2v, w));
-      }
-    }
-  }
-}
-
 template <int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, m = lane & 15;
-  const int wave = threadIdx.x >> 6;
+  // The worker index is wave-uniform; readfirstlane lets the compiler keep it
+  // (and the slice, LCG state and LDS bases derived from it) in scalar registers.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wk = blockIdx.x * a.wpb + wave;
   const int W = a.W, K = a.K, Lmax = a.Lmax;
   float* const expt = reinterpret_cast<float*>(smem);
@@ -477,7 +449,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   int plan = 0;
   bool planned = false;
   int pre_os = 0;
-  u32x4 o1[KB1], o3[4];
+  u32x4 o3[4], o3n[4];
   u32x4 in_row[2];          // rows entering the windows at this step's transition
   unsigned salt = hash32((unsigned)wk * 0x9E3779B9u + 0x6A09E667u);
 
@@ -700,7 +672,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       os = pre_os;
     } else {
       os = make_outputs(plan, t1, t2, grp);
-      load_outputs(woh, s.plans + plan * 2 * kOutRows, os, g, m, o1, o3);
+      load_output_l3(woh, s.plans + plan * 2 * kOutRows, os, g, m, o3);
     }
     const int* const cplan = s.plans + plan * 2 * kOutRows;
 
@@ -719,6 +691,9 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 
     // Contractions.
     ring_copy(S, s.rc, g, m);
+    // the output rows, row-major bf16, for GEMM1's A operand
+#pragma unroll
+    for (int e = 0; e < 4; e++) *reinterpret_cast<u32x4*>(s.ost + (4 * g + e) * RCS + 16 * m) = o3[e];
     int meta4[4];
     {
       const int4 mm = *reinterpret_cast<const int4*>(cplan + kOutRows + 4 * g);
@@ -727,12 +702,27 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     wave_sync();
     const StepLds L{s.rc, s.cb, s.mult};
     float C[3][4];
-    gemm1_coef(o1, L, meta4, os, alpha, expt, g, m, C);
+    gemm1_coef(s.ost, L, meta4, os, alpha, expt, g, m, C);
     wave_sync();
     {
+      int orank[4];
+      const int4 rr = *reinterpret_cast<const int4*>(cplan + 4 * g);
+      orank[0] = rr.x; orank[1] = rr.y; orank[2] = rr.z; orank[3] = rr.w;
+      // Woh rows of the outputs += md (bf16 atomic adds of element pairs;
+      // the lanes of rows past os add nothing)
+      typedef short __attribute__((ext_vector_type(2))) short2v;
       f32x4 md[NT];
       gemm2(S, L, g, m, md);
-      scatter_outputs(a.woh, cplan, os, g, m, md);
+      __attribute__((address_space(1))) short2v* const wb = (__attribute__((address_space(1))) short2v*)(a.woh);
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (4 * g + r < os) {
+          const unsigned base = (unsigned)orank[r] * (D / 2) + 4 * m;
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + k,
+                                                             __builtin_bit_cast(short2v, pack_rne(md[2 * k][r], md[2 * k + 1][r])));
+        }
     }
 
     // Plan the next step's outputs and read its rows (after this step's
@@ -742,10 +732,13 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     if (cont) {
       const int np = plan ^ 1;
       osn = make_outputs(np, s.tok[p0 + 1], s.tok[Lmax + p1 + 1], (neg_count + 1) % nbn);
-      load_output_l1(woh, s.plans + np * 2 * kOutRows, osn, g, m, o1);
+      load_output_l3(woh, s.plans + np * 2 * kOutRows, osn, g, m, o3n);
     }
     gemm3(S, C, o3);
-    if (cont) load_output_l3(woh, s.plans + (plan ^ 1) * 2 * kOutRows, osn, g, m, o3);
+    if (cont) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) o3[e] = o3n[e];
+    }
 
     steps++;
     n_ctx += (u64)nin;
@@ -826,10 +819,11 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha,
                                                   const float* exp_table, const float* O, const float* I,
                                                   float* corr, float* md_out, float* I_out) {
-  __shared__ __align__(16) unsigned char lds[kSlots * RCS + kOutRows * CBS + 2 * kSlots * 4 + 4096 + 64];
+  __shared__ __align__(16) unsigned char lds[(kSlots + kOutRows) * RCS + kOutRows * CBS + 2 * kSlots * 4 + 4096 + 64];
   const int lane = threadIdx.x, g = lane >> 4, m = lane & 15;
   unsigned char* rc = lds;
-  unsigned char* cb = lds + kSlots * RCS;
+  unsigned char* ost = lds + kSlots * RCS;
+  unsigned char* cb = ost + kOutRows * RCS;
   int* ml = reinterpret_cast<int*>(cb + kOutRows * CBS);
   int* plan = ml + kSlots;
   float* expt = reinterpret_cast<float*>(plan + kSlots);
@@ -844,24 +838,15 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
     for (int t = 0; t < NT; t++)
 #pragma unroll
       for (int r = 0; r < 4; r++) S.R[rt][t][r] = I[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t];
-  // Output rows as bf16, in the two register layouts.
-  u32x4 o1[KB1], o3[4];
-  {
-    const int r1 = m < os ? m : 0;
+  // Output rows as bf16 in GEMM3's layout, and row-major in LDS for GEMM1.
+  u32x4 o3[4];
 #pragma unroll
-    for (int ks = 0; ks < KB1; ks++)
+  for (int e = 0; e < 4; e++) {
+    const int i = 4 * g + e;
+    const float* src = O + (size_t)(i < os ? i : 0) * D + 8 * m;
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const float* src = O + (size_t)r1 * D + 32 * ks + 8 * g + 2 * k;
-        o1[ks][k] = pack_rne(src[0], src[1]);
-      }
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int i = 4 * g + e;
-      const float* src = O + (size_t)(i < os ? i : 0) * D + 8 * m;
-#pragma unroll
-      for (int k = 0; k < 4; k++) o3[e][k] = pack_rne(src[2 * k], src[2 * k + 1]);
-    }
+    for (int k = 0; k < 4; k++) o3[e][k] = pack_rne(src[2 * k], src[2 * k + 1]);
+    *reinterpret_cast<u32x4*>(ost + i * RCS + 16 * m) = o3[e];
   }
   __syncthreads();
   ring_copy(S, rc, g, m);
@@ -871,7 +856,7 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
   __syncthreads();
   const StepLds L{rc, cb, ml};
   float C[3][4];
-  gemm1_coef(o1, L, meta4, os, alpha, expt, g, m, C);
+  gemm1_coef(ost, L, meta4, os, alpha, expt, g, m, C);
 #pragma unroll
   for (int rt = 0; rt < 3; rt++)
 #pragma unroll

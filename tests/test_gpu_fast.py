@@ -39,12 +39,13 @@ def step_reference(Om, Im, mult, meta, alpha, table):
     lab[24:, os_ - 1] = 1
     x = (F + 6.0) * 83.0
     idx = np.clip(np.floor(x).astype(np.int64), 0, 1000)
-    # float32 arithmetic in the kernel's order: (label - e) * alpha, then
-    # times the count (contexts in the slot x output multiplicity)
+    # float32 arithmetic in the kernel's order: (label - e) * (alpha * count),
+    # count = contexts in the slot x output multiplicity; e = 1 above 6, 0
+    # below -6 (dsgl.hpp:1249-1270)
     lab32, a32 = lab.astype(np.float32), np.float32(alpha)
-    g = np.where(F > 6, (lab32 - np.float32(1)) * a32,
-                 np.where(F < -6, lab32 * a32, (lab32 - table[idx]) * a32)).astype(np.float32)
-    C = (g * (mult[:, None] * meta[None, :]).astype(np.float32)).astype(np.float64)
+    e = np.where(F > 6, np.float32(1), np.where(F < -6, np.float32(0), table[idx])).astype(np.float32)
+    cnt = (mult[:, None] * meta[None, :]).astype(np.float32)
+    C = ((lab32 - e) * (a32 * cnt)).astype(np.float32).astype(np.float64)
     # bins that float32 rounding of F could change
     amb = (np.abs(x - np.round(x)) < 1e-3 * np.maximum(1.0, np.abs(x))) | (np.abs(np.abs(F) - 6) < 1e-4)
     amb &= (mult[:, None] > 0)
@@ -106,7 +107,7 @@ def test_step_fast_layout_by_hand():
         for i in range(os_):
             lab = 1 if (i == 0 and sl < 24) or (i == os_ - 1 and sl >= 24) else 0
             idx = int((F[sl, i] + 6) * 83)
-            want = np.float32((lab - t[idx]) * mult[sl] * meta[i]) if mult[sl] else 0.0
+            want = (np.float32(lab) - t[idx]) * np.float32(mult[sl] * meta[i])
             assert corr[i, sl] == pytest.approx(want, rel=1e-6, abs=0), (sl, i)
 
 
