@@ -31,7 +31,14 @@ def test_link_scores_and_auc(precision):
         wih, _ = tr.get_model()
         s = tr.link_scores(pairs)
         a = tr.link_auc(pairs, labels)
-        oob = tr.link_scores(np.array([[0, cn.size], [-1, 3]], np.int32))
+        # ids outside [0, v_num) are rejected on the host before any upload
+        for bad_pairs in ([[0, cn.size]], [[-1, 3]]):
+            with pytest.raises(G.DgerError) as e:
+                tr.link_scores(np.array(bad_pairs, np.int32))
+            assert e.value.code == G.DGER_E_ARG
+            with pytest.raises(G.DgerError) as e:
+                tr.link_auc(np.array(bad_pairs, np.int32), np.ones(1, np.int8))
+            assert e.value.code == G.DGER_E_ARG
     by_id = np.zeros_like(wih)
     by_id[rank_id] = wih
     x, y = by_id[pairs[:, 0]].astype(np.float64), by_id[pairs[:, 1]].astype(np.float64)
@@ -43,4 +50,3 @@ def test_link_scores_and_auc(precision):
     assert abs(a - synth.auc(s, labels)) < 1e-12
     assert abs(a - synth.auc(ref, labels)) < 1e-4
     assert 0.0 <= a <= 1.0  # an ER graph has no structure to predict: about 0.5
-    assert not oob.any()

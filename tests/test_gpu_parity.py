@@ -121,12 +121,14 @@ def test_step_kernel_matches_oracle():
         assert np.array_equal(corr, c32) and np.array_equal(md, md32) and np.array_equal(gin, gin32)
         c64, md64, gin64, s64 = O.step_f64(Om, Im, fin, meta, 0.0125, t)
         _, _, _, s32 = O.step_f32(Om, Im, fin, meta, 0.0125, t)
-        flips = s32 != s64
+        flips = s32 != s64   # sigmoid-table bins that float32 rounding moved
         assert flips.mean() < 0.01
-        if not flips.any():
-            # per-step update within 1e-5 relative of the FP64 oracle
-            assert np.max(np.abs(md - md64)) <= 1e-5 * np.max(np.abs(md64))
-            assert np.max(np.abs(gin - gin64)) <= 1e-5 * np.max(np.abs(gin64))
+        # per-step update within 1e-5 relative of the FP64 oracle, on every
+        # output row (md) and context row (gin) that no flipped bin touches
+        rows, ctxs = ~flips.any(axis=1), ~flips.any(axis=0)
+        assert rows.sum() >= os_ // 2 and ctxs.sum() >= nin // 2
+        assert np.max(np.abs(md[rows] - md64[rows])) <= 1e-5 * np.max(np.abs(md64))
+        assert np.max(np.abs(gin[ctxs] - gin64[ctxs])) <= 1e-5 * np.max(np.abs(gin64))
 
 
 def test_corpus_id_out_of_range():
