@@ -61,6 +61,7 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef short short4v __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int D = kFastDim;           // row length (128)
 constexpr int NT = D / 16;            // element tiles of GEMM2 / GEMM3 (8)
@@ -68,9 +69,17 @@ constexpr int KB1 = D / 32;           // K blocks of GEMM1 (4)
 constexpr int kSlots = 48;            // 3 MFMA tiles of 16 slots
 constexpr int kSentSlots = 24;        // slots of one sentence
 constexpr int kOutRows = 16;          // output rows per step (MFMA tile)
-constexpr int RCS = 2 * D + 16;       // bytes per slot row of the LDS bf16 copy (padded against bank conflicts)
+constexpr int RB = 2 * D;             // bytes of a bf16 row in LDS
 constexpr int CBS = 144;              // bytes per output row of the transposed coefficients
+constexpr int MDS = 2 * D + 16;       // bytes per output row of the staged Woh updates
+constexpr int kMdOff = kOutRows * CBS; // staged Woh updates: in rc, after the coefficients
 constexpr int kNone = 1 << 30;
+constexpr short kNone16 = 0x7FFF;     // "no next occurrence" in the 16-bit position arrays
+
+// Byte offset of 16-byte chunk c of row r in an LDS bf16 matrix: the chunks
+// of a row are permuted by r so that sixteen lanes reading the same chunk of
+// sixteen consecutive rows hit different banks.
+__device__ __forceinline__ unsigned swz(int r, int c) { return (unsigned)r * RB + (((unsigned)(c ^ r) & 15u) << 4); }
 constexpr unsigned kRsrcWord3 = 0x00020000u;  // raw buffer descriptor word 3 for gfx9
 constexpr int kSc1 = 16;                       // cache-policy bit: coherent across XCDs (agent scope)
 
@@ -170,6 +179,23 @@ __device__ inline void occurrences(const int* v, int* first, int* prev, int* nex
   }
 }
 
+// The same with 16-bit outputs (positions are below kMaxSentence).
+__device__ inline void occurrences16(const int* v, short* first, short* prev, short* next, int n, int lane) {
+  for (int c0 = 0; c0 < n; c0 += 64) {
+    const int p = c0 + lane;
+    const int x = (p < n) ? v[p] : 0;
+    int f = p, pv = -1, nx = kNone;
+#pragma unroll 8
+    for (int q = 0; q < n; q++) {
+      const bool eq = v[q] == x;
+      f = (eq && q < f) ? q : f;
+      pv = (eq && q < p) ? q : pv;
+      nx = (eq && q > p && nx == kNone) ? q : nx;
+    }
+    if (p < n) { first[p] = (short)f; prev[p] = (short)pv; next[p] = (short)(nx == kNone ? kNone16 : nx); }
+  }
+}
+
 // ---- the slots' registers -------------------------------------------------------
 struct Slots {
   f32x4 R[3][NT];  // R[rt][t][r]: element 8m + t of slot 16 rt + 4 g + r
@@ -224,8 +250,12 @@ __device__ __forceinline__ u32x4 pack8_sr(const float (&e)[8], unsigned seed) {
 }
 
 // ---- one step's contractions ----------------------------------------------------
-// LDS of the step: rc (bf16 slot rows, RCS bytes each), cb (coefficients
-// transposed for GEMM2: [output][owner group][16 slots in K order]).
+// LDS of the step: rc (bf16 slot rows, swizzled), cb (coefficients
+// transposed for GEMM2: [output][owner group][16 slots in K order], CBS
+// bytes per output).  cb overlays the start of rc: GEMM1 reads rc before the
+// coefficients are written, and the next step's slot copy comes after GEMM2
+// has read them.  Entries 12..15 of each group hold stale finite bf16 values
+// that GEMM2 multiplies by zero.
 struct StepLds {
   unsigned char* rc;
   unsigned char* cb;
@@ -241,12 +271,12 @@ __device__ __forceinline__ void ring_copy(const Slots& S, unsigned char* rc, int
       u32x4 v;
 #pragma unroll
       for (int k = 0; k < 4; k++) v[k] = pack_rne(S.R[rt][2 * k][r], S.R[rt][2 * k + 1][r]);
-      *reinterpret_cast<u32x4*>(rc + (16 * rt + 4 * g + r) * RCS + 16 * m) = v;
+      *reinterpret_cast<u32x4*>(rc + swz(16 * rt + 4 * g + r, m)) = v;
     }
 }
 
-// GEMM1 + sigmoid + coefficients.  ost: the step's output rows in bf16,
-// RCS bytes apart (GEMM1's A operand is row m, elements 32 ks + 8 g .. +8).
+// GEMM1 + sigmoid + coefficients.  ost: the step's output rows in bf16
+// (GEMM1's A operand is row m, elements 32 ks + 8 g .. +8).
 // On return C[rt][r] = coefficient of slot 16 rt + m and
 // output 4 g + r (0 unless the slot is a context and the output exists), cb
 // holds the coefficients for GEMM2.  meta4[r] = multiplicity of output 4g+r
@@ -258,10 +288,10 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
   for (int rt = 0; rt < 3; rt++) f[rt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < KB1; ks++) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(ost + m * RCS + (32 * ks + 8 * g) * 2);
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(ost + swz(m, 4 * ks + g));
 #pragma unroll
     for (int rt = 0; rt < 3; rt++) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(L.rc + (16 * rt + m) * RCS + (32 * ks + 8 * g) * 2);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(L.rc + swz(16 * rt + m, 4 * ks + g));
       f[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, f[rt], 0, 0, 0);
     }
   }
@@ -287,13 +317,14 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
   }
 }
 
-// GEMM2: md[t][r] = the update of output row 4g + r, element 8m + t, from
-// the coefficients in cb and the slots' values before this step's update.
-__device__ __forceinline__ void gemm2(const Slots& S, const StepLds& L, int g, int m, f32x4 (&md)[NT]) {
-  const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(L.cb + m * CBS + g * 32);
-  const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(L.cb + m * CBS + g * 32 + 16);
+// GEMM2 for the element tiles t0 .. t0 + NT/2 - 1: md[t - t0][r] = the
+// update of output row 4g + r, element 8m + t, from the coefficients in cb
+// (a0, a1: GEMM2's A operand, read once per step) and the slots' values
+// before this step's update.
+__device__ __forceinline__ void gemm2_half(const Slots& S, const bf16x8& a0, const bf16x8& a1, int t0,
+                                           f32x4 (&md)[NT / 2]) {
 #pragma unroll
-  for (int t = 0; t < NT; t++) {
+  for (int t = t0; t < t0 + NT / 2; t++) {
     u32x4 b0, b1;
     b0[0] = pack_rne(S.R[0][t][0], S.R[0][t][1]);
     b0[1] = pack_rne(S.R[0][t][2], S.R[0][t][3]);
@@ -305,14 +336,17 @@ __device__ __forceinline__ void gemm2(const Slots& S, const StepLds& L, int g, i
     b1[3] = 0u;
     f32x4 c = (f32x4){0.f, 0.f, 0.f, 0.f};
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, __builtin_bit_cast(bf16x8, b0), c, 0, 0, 0);
-    md[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, __builtin_bit_cast(bf16x8, b1), c, 0, 0, 0);
+    md[t - t0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, __builtin_bit_cast(bf16x8, b1), c, 0, 0, 0);
   }
 }
 
 // GEMM3: every slot row += sum over outputs of C[slot][i] O[i] (zero for
-// slots that are no context).  o3[e]: row 4g + e of the outputs, elements
-// 8m .. 8m+7.
-__device__ __forceinline__ void gemm3(Slots& S, const float (&C)[3][4], const u32x4 (&o3)[4]) {
+// slots that are no context).  The B operand comes from the output rows in
+// ost: rows 4g .. 4g+3, elements 8m .. 8m+7.
+__device__ __forceinline__ void gemm3(Slots& S, const float (&C)[3][4], const unsigned char* ost, int g, int m) {
+  u32x4 o3[4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) o3[e] = *reinterpret_cast<const u32x4*>(ost + swz(4 * g + e, m));
   short4v a[3];
 #pragma unroll
   for (int rt = 0; rt < 3; rt++) {
@@ -332,29 +366,34 @@ __device__ __forceinline__ void gemm3(Slots& S, const float (&C)[3][4], const u3
 
 // LDS carve-up of one worker of the window-slot trainer (16-byte aligned).
 struct FastLds {
-  unsigned char* rc;  // kSlots x RCS
-  unsigned char* ost; // kOutRows x RCS: the step's output rows (bf16)
-  unsigned char* cb;  // kOutRows x CBS
+  unsigned char* rc;  // kSlots x RB: bf16 copy of the slots (swizzled); also the
+                      // staging area of a sentence's first rows, and (after GEMM1
+                      // has read it) the coefficients cb
+  unsigned char* ost; // kOutRows x RB: the step's output rows (bf16, swizzled)
+  unsigned char* cb;  // kOutRows x CBS, inside rc
   int* mult;          // kSlots
   int* srank;         // kSlots: rank held by each slot
   int* plans;         // two plans: ranks [0, 16), multiplicities [16, 32)
   int* tok;           // 2 x Lmax ranks
-  int* prv;           // 2 x Lmax
-  int* nxt;           // 2 x Lmax
-  int* sop;           // 2 x Lmax: slot of each window position
-  int* negw;          // K x Lmax negative draws
+  short* prv;         // 2 x Lmax
+  short* nxt;         // 2 x Lmax
+  short* sop;         // 2 x Lmax: slot of each window position
 };
 
 __host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
 
+static_assert(kOutRows * CBS <= kSlots * RB, "the coefficients live inside the slot copy");
+static_assert(kMdOff + kOutRows * MDS <= kSlots * RB, "the staged updates live inside the slot copy");
+
 __host__ __device__ inline size_t fast_layout(int Lmax, int K, size_t* off) {
+  (void)K;
   size_t o = 0;
-  off[0] = o; o = al16(o + (size_t)(kSlots + kOutRows) * RCS);
-  off[1] = o; o = al16(o + (size_t)kOutRows * CBS);
+  off[0] = o; o = al16(o + (size_t)(kSlots + kOutRows) * RB);
+  off[1] = o;
   off[2] = o; o = al16(o + 2 * (size_t)kSlots * 4);
   off[3] = o; o = al16(o + 2 * 2 * (size_t)kOutRows * 4);
-  off[4] = o; o = al16(o + 8 * (size_t)Lmax * 4);
-  off[5] = o; o = al16(o + (size_t)(K > 0 ? K : 1) * Lmax * 4);
+  off[4] = o; o = al16(o + 2 * (size_t)Lmax * 4);
+  off[5] = o; o = al16(o + 3 * 2 * (size_t)Lmax * 2);
   return o;
 }
 
@@ -363,17 +402,16 @@ __device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K) {
   fast_layout(Lmax, K, off);
   FastLds s;
   s.rc = base + off[0];
-  s.ost = base + off[0] + (size_t)kSlots * RCS;
-  s.cb = base + off[1];
+  s.ost = base + off[0] + (size_t)kSlots * RB;
+  s.cb = s.rc;
   s.mult = reinterpret_cast<int*>(base + off[2]);
   s.srank = s.mult + kSlots;
   s.plans = reinterpret_cast<int*>(base + off[3]);
-  int* b4 = reinterpret_cast<int*>(base + off[4]);
-  s.tok = b4;
-  s.prv = b4 + 2 * Lmax;
-  s.nxt = b4 + 4 * Lmax;
-  s.sop = b4 + 6 * Lmax;
-  s.negw = reinterpret_cast<int*>(base + off[5]);
+  s.tok = reinterpret_cast<int*>(base + off[4]);
+  short* b5 = reinterpret_cast<short*>(base + off[5]);
+  s.prv = b5;
+  s.nxt = b5 + 2 * Lmax;
+  s.sop = b5 + 4 * Lmax;
   return s;
 }
 
@@ -408,11 +446,12 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   size_t off_unused[6];
   const FastLds s = fast_carve(smem + exp_bytes() + (size_t)wave * fast_layout(Lmax, K, off_unused), Lmax, K);
   const Mat wih(a.wih, (unsigned)((size_t)a.V * D * 2)), woh(a.woh, (unsigned)((size_t)a.V * D * 2));
+  // The shared negative batch of the worker: K x Lmax draws in device memory.
+  int* const negw = reinterpret_cast<int*>(a.fscratch) + (size_t)wk * fast_scratch_ints(Lmax, K);
 
   // Slot copies and coefficients start at zero (unused slots and the K
   // padding of GEMM2 must multiply as finite zeros).
-  for (int x = lane; x < (kSlots * RCS) / 16; x += 64) reinterpret_cast<u32x4*>(s.rc)[x] = (u32x4){0u, 0u, 0u, 0u};
-  for (int x = lane; x < (kOutRows * CBS) / 16; x += 64) reinterpret_cast<u32x4*>(s.cb)[x] = (u32x4){0u, 0u, 0u, 0u};
+  for (int x = lane; x < ((kSlots + kOutRows) * RB) / 16; x += 64) reinterpret_cast<u32x4*>(s.rc)[x] = (u32x4){0u, 0u, 0u, 0u};
   if (lane < kSlots) { s.mult[lane] = 0; s.srank[lane] = 0; }
   if (lane < 2 * 2 * kOutRows) s.plans[lane] = 0;
   Slots S;
@@ -449,7 +488,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   int plan = 0;
   bool planned = false;
   int pre_os = 0;
-  u32x4 o3[4], o3n[4];
+  u32x4 o3[4];              // output rows of the next step (GEMM3 layout), on their way to ost
   u32x4 in_row[2];          // rows entering the windows at this step's transition
   unsigned salt = hash32((unsigned)wk * 0x9E3779B9u + 0x6A09E667u);
 
@@ -520,8 +559,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         len[q] = L > 1 ? L : 0;
         if (L <= 1) continue;
         wave_sync();
-        int* const sq = s.sop + q * Lmax;
-        occurrences(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, lane);
+        short* const sq = s.sop + q * Lmax;
+        occurrences16(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, lane);
         wave_sync();
         // Slots of the first window [0, W]: the k-th distinct rank takes slot 24q + k.
         const int nw = min(W, L - 1) + 1;
@@ -531,7 +570,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         int myslot = 0;
         if (lane < nw) myslot = kSentSlots * q + __popcll(nm & lanemask_lt(sq[lane]));
         wave_sync();
-        if (lane < nw) sq[lane] = myslot;
+        if (lane < nw) sq[lane] = (short)myslot;
         if (isnew) s.srank[myslot] = tq[lane];
         freem[q] = 0xFFFFFFu & ~((1u << nnew) - 1u);
         wave_sync();
@@ -585,7 +624,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
           const int n = base + 64 * c + lane;
-          if (n < NN) s.negw[n] = v[c] ? v[c] : (int)(sr[c] % (u64)(a.V - 1) + 1);
+          if (n < NN) negw[n] = v[c] ? v[c] : (int)(sr[c] % (u64)(a.V - 1) + 1);
         }
       }
       {
@@ -613,7 +652,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     const int n1 = (p1 - lo1) + (hi1 - p1);
     const int nin = n0 + n1, fin = n0;
 
-    // Rows entering the windows at this step's transition: read now.
+    // Positions entering the windows at this step's transition.
     int in_slot_src[2];  // position whose slot the entering position shares, or -1 (new row)
 #pragma unroll
     for (int q = 0; q < 2; q++) {
@@ -624,8 +663,6 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         const int pv = __builtin_amdgcn_readfirstlane(s.prv[q * Lmax + y]);
         in_slot_src[q] = (pv >= 0 && pv >= x) ? pv : -1;
       }
-      const int rk = (in_slot_src[q] == -1) ? __builtin_amdgcn_readfirstlane(s.tok[q * Lmax + y]) : 0;
-      in_row[q] = wih.load((unsigned)rk * (D * 2) + 16 * m);
     }
 
     // Context multiplicity per slot.
@@ -644,12 +681,14 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     // Output list (dsgl.hpp:1136-1158): target, distinct negatives of the
     // group with multiplicities (a negative equal to the target is dropped),
     // second target; rows past the count have multiplicity 0.
+    // v: negative k2 of the group in lane k2 < K.
     neg_count++;
     const int grp = neg_count % nbn;
-    auto make_outputs = [&](int pl, int ta, int tb, int gq) -> int {
+    // The next step's group, read now for the planning after GEMM2.
+    const int vnext = (lane < K) ? negw[((neg_count + 1) % nbn) * K + lane] : -1;
+    auto make_outputs = [&](int pl, int ta, int tb, int v) -> int {
       int* const pr = s.plans + pl * 2 * kOutRows;
       const bool live = lane < K;
-      const int v = live ? s.negw[gq * K + lane] : -1;
       int first = lane, mult = 0;
       for (int k2 = 0; k2 < K; k2++) {
         const int w2 = __builtin_amdgcn_readlane(v, k2);
@@ -671,7 +710,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     if (planned) {
       os = pre_os;
     } else {
-      os = make_outputs(plan, t1, t2, grp);
+      os = make_outputs(plan, t1, t2, (lane < K) ? negw[grp * K + lane] : -1);
       load_output_l3(woh, s.plans + plan * 2 * kOutRows, os, g, m, o3);
     }
     const int* const cplan = s.plans + plan * 2 * kOutRows;
@@ -691,9 +730,9 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 
     // Contractions.
     ring_copy(S, s.rc, g, m);
-    // the output rows, row-major bf16, for GEMM1's A operand
+    // the output rows, bf16, for GEMM1's A operand and GEMM3's B operand
 #pragma unroll
-    for (int e = 0; e < 4; e++) *reinterpret_cast<u32x4*>(s.ost + (4 * g + e) * RCS + 16 * m) = o3[e];
+    for (int e = 0; e < 4; e++) *reinterpret_cast<u32x4*>(s.ost + swz(4 * g + e, m)) = o3[e];
     int meta4[4];
     {
       const int4 mm = *reinterpret_cast<const int4*>(cplan + kOutRows + 4 * g);
@@ -705,40 +744,57 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     gemm1_coef(s.ost, L, meta4, os, alpha, expt, g, m, C);
     wave_sync();
     {
-      int orank[4];
-      const int4 rr = *reinterpret_cast<const int4*>(cplan + 4 * g);
-      orank[0] = rr.x; orank[1] = rr.y; orank[2] = rr.z; orank[3] = rr.w;
-      // Woh rows of the outputs += md (bf16 atomic adds of element pairs;
-      // the lanes of rows past os add nothing)
-      typedef short __attribute__((ext_vector_type(2))) short2v;
-      f32x4 md[NT];
-      gemm2(S, L, g, m, md);
-      __attribute__((address_space(1))) short2v* const wb = (__attribute__((address_space(1))) short2v*)(a.woh);
+      // md in bf16 pairs, staged row by row in LDS (rc is free until the
+      // next step's slot copy); then Woh row of output i += md row i with
+      // bf16 atomic adds, one row per instruction (lane l: elements 2l, 2l+1),
+      // so that each instruction covers whole 64-byte segments.
+      unsigned char* const mdl = s.rc + kMdOff;
+      const bf16x8 c0 = *reinterpret_cast<const bf16x8*>(s.cb + m * CBS + g * 32);
+      const bf16x8 c1 = *reinterpret_cast<const bf16x8*>(s.cb + m * CBS + g * 32 + 16);
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (4 * g + r < os) {
-          const unsigned base = (unsigned)orank[r] * (D / 2) + 4 * m;
+      for (int h = 0; h < 2; h++) {
+        f32x4 md[NT / 2];
+        gemm2_half(S, c0, c1, h * (NT / 2), md);
 #pragma unroll
-          for (int k = 0; k < 4; k++)
-            (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + k,
-                                                             __builtin_bit_cast(short2v, pack_rne(md[2 * k][r], md[2 * k + 1][r])));
+        for (int r = 0; r < 4; r++) {
+          u32x2 w;
+          w[0] = pack_rne(md[0][r], md[1][r]);
+          w[1] = pack_rne(md[2][r], md[3][r]);
+          *reinterpret_cast<u32x2*>(mdl + (4 * g + r) * MDS + 16 * m + 8 * h) = w;
         }
+        // keep the halves apart (their registers would not fit together)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      const int orank = (lane < kOutRows) ? cplan[lane] : 0;
+      wave_sync();
+      typedef short __attribute__((ext_vector_type(2))) short2v;
+      __attribute__((address_space(1))) short2v* const wb = (__attribute__((address_space(1))) short2v*)(a.woh);
+      for (int i = 0; i < os; i++) {
+        const unsigned v = *reinterpret_cast<const unsigned*>(mdl + i * MDS + 4 * lane);
+        const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (D / 2);
+        (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + lane, __builtin_bit_cast(short2v, v));
+      }
     }
 
+    // Wih rows entering the windows at the transition below (the loads are
+    // issued after the atomics so that the two contractions above do not
+    // hold them in registers).
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int y = (q ? p1 : p0) + W + 1;
+      const int rk = (in_slot_src[q] == -1) ? __builtin_amdgcn_readfirstlane(s.tok[q * Lmax + y]) : 0;
+      in_row[q] = wih.load((unsigned)rk * (D * 2) + 16 * m);
+    }
     // Plan the next step's outputs and read its rows (after this step's
     // updates of the same rows), when neither sentence ends here.
     const bool cont = (p0 + 1 < len[0]) && (p1 + 1 < len[1]);
     int osn = 0;
     if (cont) {
       const int np = plan ^ 1;
-      osn = make_outputs(np, s.tok[p0 + 1], s.tok[Lmax + p1 + 1], (neg_count + 1) % nbn);
-      load_output_l3(woh, s.plans + np * 2 * kOutRows, osn, g, m, o3n);
+      osn = make_outputs(np, s.tok[p0 + 1], s.tok[Lmax + p1 + 1], vnext);
+      load_output_l3(woh, s.plans + np * 2 * kOutRows, osn, g, m, o3);
     }
-    gemm3(S, C, o3);
-    if (cont) {
-#pragma unroll
-      for (int e = 0; e < 4; e++) o3[e] = o3n[e];
-    }
+    gemm3(S, C, s.ost, g, m);
 
     steps++;
     n_ctx += (u64)nin;
@@ -751,7 +807,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       const int p = pos[q];
       const int Lq = len[q];
       int* const tq = s.tok + q * Lmax;
-      int* const sq = s.sop + q * Lmax;
+      short* const sq = s.sop + q * Lmax;
       pos[q] = p + 1;
       if (p + 1 >= Lq) {
         for (unsigned live = ~freem[q] & 0xFFFFFFu; live; live &= live - 1) {
@@ -781,13 +837,13 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       }
       if (in_slot_src[q] >= 0) {
         const int sy = __builtin_amdgcn_readfirstlane(sq[in_slot_src[q]]);
-        if (lane == 0) sq[y] = sy;
+        if (lane == 0) sq[y] = (short)sy;
       } else if (in_slot_src[q] == -1) {
         const int k = __ffs((int)freem[q]) - 1;
         freem[q] &= ~(1u << k);
         const int sl = kSentSlots * q + k;
         slot_put(S, sl, in_row[q], g);
-        if (lane == 0) { sq[y] = sl; s.srank[sl] = tq[y]; }
+        if (lane == 0) { sq[y] = (short)sl; s.srank[sl] = tq[y]; }
       }
     }
     if (done_sent >= 2) {
@@ -819,16 +875,15 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha,
                                                   const float* exp_table, const float* O, const float* I,
                                                   float* corr, float* md_out, float* I_out) {
-  __shared__ __align__(16) unsigned char lds[(kSlots + kOutRows) * RCS + kOutRows * CBS + 2 * kSlots * 4 + 4096 + 64];
+  __shared__ __align__(16) unsigned char lds[(kSlots + kOutRows) * RB + 2 * kSlots * 4 + 4096 + 64];
   const int lane = threadIdx.x, g = lane >> 4, m = lane & 15;
   unsigned char* rc = lds;
-  unsigned char* ost = lds + kSlots * RCS;
-  unsigned char* cb = ost + kOutRows * RCS;
-  int* ml = reinterpret_cast<int*>(cb + kOutRows * CBS);
+  unsigned char* ost = lds + kSlots * RB;
+  unsigned char* cb = rc;
+  int* ml = reinterpret_cast<int*>(ost + kOutRows * RB);
   int* plan = ml + kSlots;
   float* expt = reinterpret_cast<float*>(plan + kSlots);
   for (int x = lane; x < kExpTable; x += 64) expt[x] = exp_table[x];
-  for (int x = lane; x < (kOutRows * CBS) / 16; x += 64) reinterpret_cast<u32x4*>(cb)[x] = (u32x4){0u, 0u, 0u, 0u};
   if (lane < kSlots) ml[lane] = mult[lane];
   if (lane < kOutRows) { plan[lane] = lane; plan[kOutRows + lane] = lane < os ? meta[lane] : 0; }
   Slots S;
@@ -838,15 +893,15 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
     for (int t = 0; t < NT; t++)
 #pragma unroll
       for (int r = 0; r < 4; r++) S.R[rt][t][r] = I[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t];
-  // Output rows as bf16 in GEMM3's layout, and row-major in LDS for GEMM1.
-  u32x4 o3[4];
+  // Output rows as bf16 in LDS (rows past os repeat row 0).
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     const int i = 4 * g + e;
     const float* src = O + (size_t)(i < os ? i : 0) * D + 8 * m;
+    u32x4 o;
 #pragma unroll
-    for (int k = 0; k < 4; k++) o3[e][k] = pack_rne(src[2 * k], src[2 * k + 1]);
-    *reinterpret_cast<u32x4*>(ost + i * RCS + 16 * m) = o3[e];
+    for (int k = 0; k < 4; k++) o[k] = pack_rne(src[2 * k], src[2 * k + 1]);
+    *reinterpret_cast<u32x4*>(ost + swz(i, m)) = o;
   }
   __syncthreads();
   ring_copy(S, rc, g, m);
@@ -863,14 +918,19 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
     for (int r = 0; r < 4; r++)
       if (4 * g + r < os) corr[(size_t)(4 * g + r) * kSlots + 16 * rt + m] = C[rt][r];
   __syncthreads();
-  f32x4 md[NT];
-  gemm2(S, L, g, m, md);
+  const bf16x8 c0 = *reinterpret_cast<const bf16x8*>(cb + m * CBS + g * 32);
+  const bf16x8 c1 = *reinterpret_cast<const bf16x8*>(cb + m * CBS + g * 32 + 16);
 #pragma unroll
-  for (int r = 0; r < 4; r++)
-    if (4 * g + r < os)
+  for (int h = 0; h < 2; h++) {
+    f32x4 md[NT / 2];
+    gemm2_half(S, c0, c1, h * (NT / 2), md);
 #pragma unroll
-      for (int t = 0; t < NT; t++) md_out[(size_t)(4 * g + r) * D + 8 * m + t] = md[t][r];
-  gemm3(S, C, o3);
+    for (int r = 0; r < 4; r++)
+      if (4 * g + r < os)
+#pragma unroll
+        for (int t = 0; t < NT / 2; t++) md_out[(size_t)(4 * g + r) * D + 8 * m + h * (NT / 2) + t] = md[t][r];
+  }
+  gemm3(S, C, ost, g, m);
 #pragma unroll
   for (int rt = 0; rt < 3; rt++)
 #pragma unroll

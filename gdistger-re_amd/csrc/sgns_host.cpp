@@ -472,14 +472,15 @@ int dger_ctx_set_corpus_walks(dger_ctx* c, const dger_walk_corpus* w) {
 }
 
 // Default worker count: as many workgroups per CU as its LDS holds, at most
-// one wavefront per SIMD (the trainer uses more than 256 registers).
+// one wavefront per SIMD for the general trainer (it uses more than 256
+// registers) and two for the window-slot trainer.
 static int auto_workers(dger_ctx* c, size_t block_lds, int wpb) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->o.device) != hipSuccess) return 1024;
   const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
                                                               : (size_t)160 * 1024;
   int per_cu = (int)std::max<size_t>(1, lds_cu / std::max<size_t>(block_lds, 1)) * wpb;
-  per_cu = std::min(per_cu, 4);
+  per_cu = std::min(per_cu, c->fast ? dger::kFastMaxWpb : 4);
   int w = prop.multiProcessorCount * per_cu;
   // Keep slices long enough that dropped in-flight sentences stay rare.
   const uint64_t max_by_corpus = std::max<uint64_t>(1, c->n_tokens / 4096);
@@ -488,7 +489,8 @@ static int auto_workers(dger_ctx* c, size_t block_lds, int wpb) {
 }
 
 static int ensure_worker_buffers(dger_ctx* c, int workers) {
-  const size_t per = c->fast ? 0 : dger::worker_scratch_floats(c->Lmax, c->o.negative, c->D);
+  const size_t per = c->fast ? dger::fast_scratch_ints(c->Lmax, c->o.negative)
+                             : dger::worker_scratch_floats(c->Lmax, c->o.negative, c->D);
   if (c->scratch_d.ensure((size_t)workers * per * 4) || c->wsteps_d.ensure((size_t)workers * 8) ||
       c->walpha_d.ensure((size_t)workers * 4))
     return fail(DGER_E_ALLOC, "scratch allocation failed");
@@ -576,8 +578,8 @@ static int run_pass(dger_ctx* c, const dger::TrainArgs& a, bool use_graph, float
 }
 
 // Worker-count autotuner: times one pass over a corpus prefix for one to four
-// workers per CU (one workgroup per CU) and returns the fastest in trained
-// words per second.  The model is saved before and restored after.
+// workers per CU (eight for the window-slot trainer; one workgroup per CU) and
+// returns the fastest in trained words per second.  The model is saved before and restored after.
 static int autotune_workers(dger_ctx* c, int cus, int wpb_max, int* out) {
   DevBuf* model[4] = {&c->wih_d, &c->woh_d, &c->wm_d, &c->om_d};
   DevBuf save[4];
@@ -589,7 +591,7 @@ static int autotune_workers(dger_ctx* c, int cus, int wpb_max, int* out) {
   const uint64_t sample = std::min<uint64_t>(c->n_tokens, 16ull << 20);
   double best = -1.0;
   int best_w = 0;
-  for (int per_cu = 1; per_cu <= std::min(4, wpb_max); per_cu++) {
+  for (int per_cu = 1; per_cu <= std::min(c->fast ? dger::kFastMaxWpb : 4, wpb_max); per_cu++) {
     const int w = std::max<int>(1, std::min<uint64_t>((uint64_t)cus * per_cu, std::max<uint64_t>(1, sample / 4096)));
     if (const int rc = ensure_worker_buffers(c, w); rc != DGER_OK) return rc;
     const dger::TrainArgs a = make_args(c, w, std::min(per_cu, w), nullptr, 0, sample);

@@ -81,9 +81,12 @@ DGER_HD inline int record_stride(int W, int K) { return 8 + 4 * W + 2 * (K + 2);
 // Window-slot trainer of the BF16-MFMA form (sgns_fast.hip): bf16 rows of
 // length kFastDim, window slots in registers, contractions on MFMA.
 constexpr int kFastDim = 128;
-constexpr int kFastMaxWpb = 4;  // workers (wavefronts) per workgroup: one per SIMD
+constexpr int kFastMaxWpb = 8;  // workers (wavefronts) per workgroup: two per SIMD
 bool fast_supported(int D, int K, int W, uint32_t V);
 size_t fast_lds_bytes(int Lmax, int K);
+// Device memory per worker of the window-slot trainer, in 4-byte words: its
+// shared negative batch (K x Lmax ranks).
+DGER_HD inline size_t fast_scratch_ints(int Lmax, int K) { return (size_t)(K > 0 ? K : 1) * (size_t)Lmax; }
 size_t fast_block_lds_bytes(int Lmax, int K, int wpb);
 int fast_workers_per_block(int Lmax, int K, size_t lds_per_block);
 hipError_t prepare_train_fast(const TrainArgs& a);
