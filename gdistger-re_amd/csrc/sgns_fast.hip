@@ -295,12 +295,23 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
       f[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, f[rt], 0, 0, 0);
     }
   }
-  // f[rt][r] = F[slot 16 rt + m][output 4 g + r]
+  // f[rt][r] = F[slot 16 rt + m][output 4 g + r].  All table reads are
+  // issued before the coefficients are stored (the stores could alias them).
+  float tab[3][4];
+  int mu[3];
 #pragma unroll
   for (int rt = 0; rt < 3; rt++) {
-    const int sl = 16 * rt + m;
-    const int mu = L.mult[sl];
-    const bool second = sl >= kSentSlots;
+    mu[rt] = L.mult[16 * rt + m];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      int idx = (int)((f[rt][r] + 6.0f) * 83.0f);
+      idx = min(max(idx, 0), kExpTable - 1);
+      tab[rt][r] = expt[idx];
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++) {
+    const bool second = 16 * rt + m >= kSentSlots;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int i = 4 * g + r;
@@ -308,13 +319,15 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
       // (label - sigmoid) * alpha * count, branch-free: a count of 0 (no
       // context in the slot, or an output past os) gives exactly 0
       const float x = f[rt][r];
-      int idx = (int)((x + 6.0f) * 83.0f);
-      idx = min(max(idx, 0), kExpTable - 1);
-      const float e = (x > 6.0f) ? 1.f : ((x < -6.0f) ? 0.f : expt[idx]);
-      C[rt][r] = (lab - e) * (alpha * (float)(meta4[r] * mu));
-      *reinterpret_cast<__bf16*>(L.cb + i * CBS + (m >> 2) * 32 + (4 * rt + (m & 3)) * 2) = (__bf16)C[rt][r];
+      const float e = (x > 6.0f) ? 1.f : ((x < -6.0f) ? 0.f : tab[rt][r]);
+      C[rt][r] = (lab - e) * (alpha * (float)(meta4[r] * mu[rt]));
     }
   }
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      *reinterpret_cast<__bf16*>(L.cb + (4 * g + r) * CBS + (m >> 2) * 32 + (4 * rt + (m & 3)) * 2) = (__bf16)C[rt][r];
 }
 
 // GEMM2 for the element tiles t0 .. t0 + NT/2 - 1: md[t - t0][r] = the
@@ -704,7 +717,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       if (keep) { pr[o] = v; pr[kOutRows + o] = mult; }
       if (lane == 0) { pr[0] = ta; pr[kOutRows] = 1; pr[n - 1] = tb; pr[kOutRows + n - 1] = 1; }
       wave_sync();
-      return n;
+      return __builtin_amdgcn_readfirstlane(n);  // the same in every lane
     };
     int os;
     if (planned) {
@@ -793,6 +806,11 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       const int np = plan ^ 1;
       osn = make_outputs(np, s.tok[p0 + 1], s.tok[Lmax + p1 + 1], vnext);
       load_output_l3(woh, s.plans + np * 2 * kOutRows, osn, g, m, o3);
+    } else {
+      // the next step plans afresh: the rows are dead (and need no register
+      // across this step)
+#pragma unroll
+      for (int e = 0; e < 4; e++) o3[e] = (u32x4){0u, 0u, 0u, 0u};
     }
     gemm3(S, C, s.ost, g, m);
 
