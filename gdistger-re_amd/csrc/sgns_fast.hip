@@ -196,6 +196,42 @@ __device__ inline void occurrences16(const int* v, short* first, short* prev, sh
   }
 }
 
+// ---- remainders ----------------------------------------------------------------
+// Remainders by divisors that stay fixed for many uses, without a division
+// each time.  Div32: M = floor((2^32 - 1) / d), so that for every 32-bit x,
+// mulhi(x, M) is floor(x / d) or one less (M d > 2^32 - 1 - d) and one
+// conditional subtraction finishes the remainder.
+struct Div32 {
+  unsigned d, M;
+};
+__device__ __forceinline__ Div32 make_div32(unsigned d) { return Div32{d, 0xFFFFFFFFu / d}; }
+__device__ __forceinline__ unsigned mod32(unsigned x, const Div32& v) {
+  const unsigned r = x - __umulhi(x, v.M) * v.d;
+  return r >= v.d ? r - v.d : r;
+}
+// x mod d for a 64-bit x and d < 2^16: long division by 16-bit digits.
+__device__ __forceinline__ unsigned mod64_small(u64 x, const Div32& v) {
+  unsigned r = mod32((unsigned)(x >> 32), v);
+  r = mod32((r << 16) | (unsigned)((x >> 16) & 0xFFFFu), v);
+  return mod32((r << 16) | (unsigned)(x & 0xFFFFu), v);
+}
+// x mod d from inv = 1 / d rounded to double, for x < 2^63 and x / d < 2^51:
+// x * inv is within 3 * 2^-53 * x / d < 1 of x / d, so its floor is the
+// quotient or one off, and one correction step finishes the remainder.
+__device__ __forceinline__ u64 mod_recip(u64 x, unsigned d, double inv) {
+  const u64 q = (u64)__builtin_floor((double)x * inv);
+  long long r = (long long)(x - q * (u64)d);
+  r += (r < 0) ? (long long)d : 0;
+  r -= (r >= (long long)d) ? (long long)d : 0;
+  return (u64)r;
+}
+// x mod d for any 64-bit x and d < 2^31: the high word first, then the
+// 64-bit value below d * 2^32 that remains.
+__device__ __forceinline__ unsigned mod64_recip(u64 x, unsigned d, double inv) {
+  const u64 h = mod_recip(x >> 32, d, inv);
+  return (unsigned)mod_recip((h << 32) | (x & 0xFFFFFFFFull), d, inv);
+}
+
 // ---- the slots' registers -------------------------------------------------------
 struct Slots {
   f32x4 R[3][NT];  // R[rt][t][r]: element 8m + t of slot 16 rt + 4 g + r
@@ -495,6 +531,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   int len[2] = {0, 0}, pos[2] = {0, 0};
   unsigned freem[2] = {0xFFFFFFu, 0xFFFFFFu};  // free slots of each sentence (bit k: slot 24q + k)
   int done_sent = 0, refresh = 1, neg_count = 0, nbn = 1;
+  const Div32 divW = make_div32((unsigned)W);
+  Div32 divN = make_div32(1u);
   u64 steps = 0, raw = 0, sentences = 0, n_ctx = 0, n_outctx = 0;
   const int stride = record_stride(W, K);
   int32_t* const rec = a.rec ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
@@ -508,10 +546,11 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   // Learning-rate decay, sentence loads, slice end, refresh: as the reference
   // loop (dsgl.hpp:886-1088).
   for (;;) {
-    // Lane-derived values are recomputed in every iteration from an opaque
-    // copy of the lane index, instead of being kept live across the loop.
-    int lane_o = threadIdx.x & 63;
-    asm volatile("" : "+v"(lane_o));
+    // Lane-derived values are recomputed in every iteration from the lane
+    // index, counted from the full execution mask in volatile asm so that it
+    // is neither hoisted out of the loop nor kept live across it.
+    int lane_o;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane_o));
     const int lane = lane_o, g = lane_o >> 4, m = lane_o & 15;
     if (wc - last_wc > 10000) {
       const u64 diff = wc - last_wc;
@@ -617,7 +656,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     // Shared negative batch (dsgl.hpp:1054-1080): K * max(L1, L2) draws.
     if (refresh) {
       nbn = max(len[0], len[1]);
+      divN = make_div32((unsigned)nbn);
       const int NN = nbn * K;
+      const unsigned ts = (unsigned)a.table_size, vm1 = (unsigned)(a.V - 1);
+      const double inv_ts = 1.0 / (double)ts, inv_vm1 = 1.0 / (double)vm1;
       const u64 A64 = readlane64(jA, 63), C64 = readlane64(jC, 63);
       const u64 r0 = rng;
       u64 cs = r0;
@@ -632,12 +674,12 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
           const int n = base + 64 * c + lane;
-          v[c] = (n < NN) ? a.table[(sr[c] >> 16) % a.table_size] : 1;
+          v[c] = (n < NN) ? a.table[mod_recip(sr[c] >> 16, ts, inv_ts)] : 1;
         }
 #pragma unroll
         for (int c = 0; c < 8; c++) {
           const int n = base + 64 * c + lane;
-          if (n < NN) negw[n] = v[c] ? v[c] : (int)(sr[c] % (u64)(a.V - 1) + 1);
+          if (n < NN) negw[n] = v[c] ? v[c] : (int)(mod64_recip(sr[c], vm1, inv_vm1) + 1);
         }
       }
       {
@@ -657,7 +699,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     const int t1 = __builtin_amdgcn_readfirstlane(s.tok[p0]);
     const int t2 = __builtin_amdgcn_readfirstlane(s.tok[Lmax + p1]);
     rng = rng * kLcgMul + kLcgAdd;
-    const int b = (int)(rng % (u64)W);
+    const int b = (int)mod64_small(rng, divW);
     const int hw = W - b;
     const int lo0 = max(0, p0 - hw), hi0 = min(len[0] - 1, p0 + hw);
     const int lo1 = max(0, p1 - hw), hi1 = min(len[1] - 1, p1 + hw);
@@ -696,9 +738,9 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     // second target; rows past the count have multiplicity 0.
     // v: negative k2 of the group in lane k2 < K.
     neg_count++;
-    const int grp = neg_count % nbn;
+    const int grp = (int)mod32((unsigned)neg_count, divN);
     // The next step's group, read now for the planning after GEMM2.
-    const int vnext = (lane < K) ? negw[((neg_count + 1) % nbn) * K + lane] : -1;
+    const int vnext = (lane < K) ? negw[(int)mod32((unsigned)(neg_count + 1), divN) * K + lane] : -1;
     auto make_outputs = [&](int pl, int ta, int tb, int v) -> int {
       int* const pr = s.plans + pl * 2 * kOutRows;
       const bool live = lane < K;
@@ -869,7 +911,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       done_sent = 0;
     }
     planned = cont;
-    if (cont) { plan ^= 1; pre_os = osn; }
+    if (cont) { plan = __builtin_amdgcn_readfirstlane(plan ^ 1); pre_os = osn; }
     salt = hash32(salt + 0x9E3779B9u);
     wave_sync();
   }
@@ -959,9 +1001,9 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
 
 }  // namespace
 
-bool fast_supported(int D_, int K, int W, uint32_t V) {
-  return D_ == D && K + 2 <= kOutRows && 2 * W + 1 <= kSentSlots && W >= 1 &&
-         (uint64_t)V * D * 2 < (1ull << 31);
+bool fast_supported(int D_, int K, int W, uint32_t V, int64_t table_size) {
+  return D_ == D && K + 2 <= kOutRows && 2 * W + 1 <= kSentSlots && W >= 1 && V >= 2 &&
+         (uint64_t)V * D * 2 < (1ull << 31) && table_size >= 1 && table_size < (1ll << 31);
 }
 
 size_t fast_lds_bytes(int Lmax, int K) {

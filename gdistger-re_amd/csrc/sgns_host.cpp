@@ -367,7 +367,7 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   c->V = v_num;
   c->D = opts->size;
   c->fast = opts->precision == DGER_PRECISION_BF16_MFMA &&
-            dger::fast_supported(opts->size, opts->negative, opts->window, v_num);
+            dger::fast_supported(opts->size, opts->negative, opts->window, v_num, opts->table_size);
   auto bail = [&](int code) { delete c; return code; };
   if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess)

@@ -82,7 +82,7 @@ DGER_HD inline int record_stride(int W, int K) { return 8 + 4 * W + 2 * (K + 2);
 // length kFastDim, window slots in registers, contractions on MFMA.
 constexpr int kFastDim = 128;
 constexpr int kFastMaxWpb = 8;  // workers (wavefronts) per workgroup: two per SIMD
-bool fast_supported(int D, int K, int W, uint32_t V);
+bool fast_supported(int D, int K, int W, uint32_t V, int64_t table_size);
 size_t fast_lds_bytes(int Lmax, int K);
 // Device memory per worker of the window-slot trainer, in 4-byte words: its
 // shared negative batch (K x Lmax ranks).
