@@ -75,6 +75,7 @@ constexpr int MDS = 2 * D + 16;       // bytes per output row of the staged Woh 
 constexpr int kMdOff = kOutRows * CBS; // staged Woh updates: in rc, after the coefficients
 constexpr int kNone = 1 << 30;
 constexpr short kNone16 = 0x7FFF;     // "no next occurrence" in the 16-bit position arrays
+constexpr int kExpOne = kExpTable, kExpZero = kExpTable + 1;  // sigmoid 1 and 0 after the table
 
 // Byte offset of 16-byte chunk c of row r in an LDS bf16 matrix: the chunks
 // of a row are permuted by r so that sixteen lanes reading the same chunk of
@@ -331,17 +332,23 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
       f[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, f[rt], 0, 0, 0);
     }
   }
-  // f[rt][r] = F[slot 16 rt + m][output 4 g + r].  All table reads are
-  // issued before the coefficients are stored (the stores could alias them).
-  float tab[3][4];
-  int mu[3];
+  // f[rt][r] = F[slot 16 rt + m][output 4 g + r].  The sigmoid is one
+  // table read: the bin of x, or the entries 1 (x > 6) and 0 (x < -6) after
+  // the table, so that no branch is needed.  All reads are issued before the
+  // coefficients are stored (the stores could alias them).
+  float tab[3][4], mu[3], me[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) me[r] = (float)meta4[r];
 #pragma unroll
   for (int rt = 0; rt < 3; rt++) {
-    mu[rt] = L.mult[16 * rt + m];
+    mu[rt] = (float)L.mult[16 * rt + m];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      int idx = (int)((f[rt][r] + 6.0f) * 83.0f);
+      const float x = f[rt][r];
+      int idx = (int)((x + 6.0f) * 83.0f);
       idx = min(max(idx, 0), kExpTable - 1);
+      idx = (x > 6.0f) ? kExpOne : idx;
+      idx = (x < -6.0f) ? kExpZero : idx;
       tab[rt][r] = expt[idx];
     }
   }
@@ -352,11 +359,10 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
     for (int r = 0; r < 4; r++) {
       const int i = 4 * g + r;
       const float lab = (second ? (i == os - 1) : (i == 0)) ? 1.f : 0.f;
-      // (label - sigmoid) * alpha * count, branch-free: a count of 0 (no
-      // context in the slot, or an output past os) gives exactly 0
-      const float x = f[rt][r];
-      const float e = (x > 6.0f) ? 1.f : ((x < -6.0f) ? 0.f : tab[rt][r]);
-      C[rt][r] = (lab - e) * (alpha * (float)(meta4[r] * mu[rt]));
+      // (label - sigmoid) * alpha * count: a count of 0 (no context in the
+      // slot, or an output past os) gives exactly 0; the count (a product of
+      // small integers) is exact in float
+      C[rt][r] = (lab - tab[rt][r]) * (alpha * (me[r] * mu[rt]));
     }
   }
 #pragma unroll
@@ -464,7 +470,13 @@ __device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K) {
   return s;
 }
 
-__host__ __device__ inline size_t exp_bytes() { return al16((size_t)kExpTable * 4); }
+// The sigmoid table in LDS: the kExpTable entries, then 1 and 0 (the values
+// above and below the table's range, at kExpOne and kExpZero).
+__host__ __device__ inline size_t exp_bytes() { return al16((size_t)(kExpTable + 2) * 4); }
+__device__ inline void load_exp_table(float* expt, const float* src, int tid, int nthreads) {
+  for (int x = tid; x < kExpTable; x += nthreads) expt[x] = src[x];
+  if (tid == 0) { expt[kExpOne] = 1.f; expt[kExpZero] = 0.f; }
+}
 
 // Output rows of a plan for GEMM3's B operand (rows 4g .. 4g+3, elements
 // 8m .. 8m+7); rows past os read row 0 of the plan (their results are not
@@ -489,7 +501,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   const int wk = blockIdx.x * a.wpb + wave;
   const int W = a.W, K = a.K, Lmax = a.Lmax;
   float* const expt = reinterpret_cast<float*>(smem);
-  for (int x = threadIdx.x; x < kExpTable; x += blockDim.x) expt[x] = a.exp_table[x];
+  load_exp_table(expt, a.exp_table, threadIdx.x, blockDim.x);
   __syncthreads();
   if (wk >= a.workers) return;
   size_t off_unused[6];
@@ -943,7 +955,7 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
   int* ml = reinterpret_cast<int*>(ost + kOutRows * RB);
   int* plan = ml + kSlots;
   float* expt = reinterpret_cast<float*>(plan + kSlots);
-  for (int x = lane; x < kExpTable; x += 64) expt[x] = exp_table[x];
+  load_exp_table(expt, exp_table, lane, 64);
   if (lane < kSlots) ml[lane] = mult[lane];
   if (lane < kOutRows) { plan[lane] = lane; plan[kOutRows + lane] = lane < os ? meta[lane] : 0; }
   Slots S;
