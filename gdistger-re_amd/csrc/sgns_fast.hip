@@ -419,6 +419,13 @@ __device__ __forceinline__ void gemm3(Slots& S, const float (&C)[3][4], const un
   }
 }
 
+// Per-worker totals in LDS: contexts, output-context pairs, sentences, raw words.
+enum { kStatCtx = 0, kStatOutCtx, kStatSentences, kStatRaw, kStatCount };
+// One lane adds v to a per-worker total (an LDS atomic without return).
+__device__ __forceinline__ void stat_add(u64* stats, int k, u64 v) {
+  __hip_atomic_fetch_add(stats + k, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 // LDS carve-up of one worker of the window-slot trainer (16-byte aligned).
 struct FastLds {
   unsigned char* rc;  // kSlots x RB: bf16 copy of the slots (swizzled); also the
@@ -433,6 +440,7 @@ struct FastLds {
   short* prv;         // 2 x Lmax
   short* nxt;         // 2 x Lmax
   short* sop;         // 2 x Lmax: slot of each window position
+  u64* stats;         // kStatCount per-worker totals (kept here, not in scalar registers)
 };
 
 __host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -449,6 +457,7 @@ __host__ __device__ inline size_t fast_layout(int Lmax, int K, size_t* off) {
   off[3] = o; o = al16(o + 2 * 2 * (size_t)kOutRows * 4);
   off[4] = o; o = al16(o + 2 * (size_t)Lmax * 4);
   off[5] = o; o = al16(o + 3 * 2 * (size_t)Lmax * 2);
+  off[1] = o; o = al16(o + kStatCount * 8);
   return o;
 }
 
@@ -467,6 +476,7 @@ __device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K) {
   s.prv = b5;
   s.nxt = b5 + 2 * Lmax;
   s.sop = b5 + 4 * Lmax;
+  s.stats = reinterpret_cast<u64*>(base + off[1]);
   return s;
 }
 
@@ -514,6 +524,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   // padding of GEMM2 must multiply as finite zeros).
   for (int x = lane; x < ((kSlots + kOutRows) * RB) / 16; x += 64) reinterpret_cast<u32x4*>(s.rc)[x] = (u32x4){0u, 0u, 0u, 0u};
   if (lane < kSlots) { s.mult[lane] = 0; s.srank[lane] = 0; }
+  if (lane < kStatCount) s.stats[lane] = 0;
   if (lane < 2 * 2 * kOutRows) s.plans[lane] = 0;
   Slots S;
 #pragma unroll
@@ -545,7 +556,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   int done_sent = 0, refresh = 1, neg_count = 0, nbn = 1;
   const Div32 divW = make_div32((unsigned)W);
   Div32 divN = make_div32(1u);
-  u64 steps = 0, raw = 0, sentences = 0, n_ctx = 0, n_outctx = 0;
+  u64 steps = 0;
   const int stride = record_stride(W, K);
   int32_t* const rec = a.rec ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
   int plan = 0;
@@ -656,7 +667,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     // End of the slice: in-flight sentences are dropped (dsgl.hpp:1031-1050).
     if (cur >= end) {
       if (lane == 0) atomicAdd(&a.counters[kWordCountActual], wc - last_wc);
-      raw += wc;
+      if (lane == 0) stat_add(s.stats, kStatRaw, wc);
       if (--iters_left == 0) break;
       wc = 0; cur = start; last_wc = 0;
       len[0] = 0; len[1] = 0;
@@ -869,8 +880,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     gemm3(S, C, s.ost, g, m);
 
     steps++;
-    n_ctx += (u64)nin;
-    n_outctx += (u64)(os * nin);
+    if (lane == 0) {
+      stat_add(s.stats, kStatCtx, (u64)nin);
+      stat_add(s.stats, kStatOutCtx, (u64)(os * nin));
+    }
 
     // Transition: a finished sentence writes its live slots back; a
     // continuing one slides its window by one position.
@@ -893,7 +906,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         freem[q] = 0xFFFFFFu;
         len[q] = 0;
         done_sent++;
-        sentences++;
+        if (lane == 0) stat_add(s.stats, kStatSentences, 1);
         continue;
       }
       const int x = p - W, y = p + W + 1;
@@ -932,10 +945,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     a.worker_steps[wk] = steps;
     a.worker_alpha[wk] = alpha;
     atomicAdd(&a.counters[kSteps], steps);
-    atomicAdd(&a.counters[kRawWords], raw);
-    atomicAdd(&a.counters[kSentences], sentences);
-    atomicAdd(&a.counters[kContexts], n_ctx);
-    atomicAdd(&a.counters[kOutCtx], n_outctx);
+    atomicAdd(&a.counters[kRawWords], s.stats[kStatRaw]);
+    atomicAdd(&a.counters[kSentences], s.stats[kStatSentences]);
+    atomicAdd(&a.counters[kContexts], s.stats[kStatCtx]);
+    atomicAdd(&a.counters[kOutCtx], s.stats[kStatOutCtx]);
   }
 }
 
