@@ -106,9 +106,6 @@ __device__ __forceinline__ unsigned pack_rne(float a, float b) {
   return __builtin_bit_cast(unsigned, v);
 }
 // Stochastic rounding of finite x to bf16 with 16 random bits.
-__device__ __forceinline__ unsigned bf16_sr(float x, unsigned rnd) {
-  return (__float_as_uint(x) + (rnd & 0xFFFFu)) >> 16;
-}
 __device__ __forceinline__ void unpack8(const u32x4& v, float (&e)[8]) {
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -274,13 +271,19 @@ __device__ __forceinline__ void slot_get(const Slots& S, int s, float (&e)[8]) {
   }
 }
 
-// Eight floats to bf16 with stochastic rounding (random bits from seed).
+// Eight floats to bf16 with stochastic rounding (random bits from seed):
+// v_cvt_sr_bf16_f32 adds the high 16 bits of its random operand to the
+// float's bits and truncates (NaN gives the canonical NaN, denormals zero),
+// so each pair takes the two halves of one xorshift state.
 __device__ __forceinline__ u32x4 pack8_sr(const float (&e)[8], unsigned seed) {
   u32x4 w;
   unsigned h = hash32(seed);
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    w[k] = bf16_sr(e[2 * k], h) | (bf16_sr(e[2 * k + 1], h >> 16) << 16);
+    bf16x2 v = {};
+    v = __builtin_amdgcn_cvt_sr_bf16_f32(v, e[2 * k], h, false);
+    v = __builtin_amdgcn_cvt_sr_bf16_f32(v, e[2 * k + 1], h << 16, true);
+    w[k] = __builtin_bit_cast(unsigned, v);
     h ^= h << 13; h ^= h >> 17; h ^= h << 5;
   }
   return w;
@@ -563,7 +566,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   bool planned = false;
   int pre_os = 0;
   u32x4 o3[4];              // output rows of the next step (GEMM3 layout), on their way to ost
-  u32x4 in_row[2];          // rows entering the windows at this step's transition
+  u32x4 in_row[2];          // rows entering the windows at a transition
+  int pend[2] = {-1, -1};   // their slots, filled at the next step's start
   unsigned salt = hash32((unsigned)wk * 0x9E3779B9u + 0x6A09E667u);
 
   // Learning-rate decay, sentence loads, slice end, refresh: as the reference
@@ -672,6 +676,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       wc = 0; cur = start; last_wc = 0;
       len[0] = 0; len[1] = 0;
       freem[0] = 0xFFFFFFu; freem[1] = 0xFFFFFFu;
+      pend[0] = -1; pend[1] = -1;
       planned = false;
       continue;
     }
@@ -807,6 +812,15 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     }
 
     // Contractions.
+    // Rows that entered the windows at the last transition take their slots
+    // now (their loads, issued during that step, are waited for here
+    // together with this step's output rows).
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+      if (pend[q] >= 0) {
+        slot_put(S, pend[q], in_row[q], g);
+        pend[q] = -1;
+      }
     ring_copy(S, s.rc, g, m);
     // the output rows, bf16, for GEMM1's A operand and GEMM3's B operand
 #pragma unroll
@@ -854,9 +868,9 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       }
     }
 
-    // Wih rows entering the windows at the transition below (the loads are
-    // issued after the atomics so that the two contractions above do not
-    // hold them in registers).
+    // Wih rows entering the windows at the transition below; they take their
+    // slots at the start of the next step (the loads are issued after the
+    // atomics so that the contractions above do not hold them in registers).
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int y = (q ? p1 : p0) + W + 1;
@@ -927,7 +941,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         const int k = __ffs((int)freem[q]) - 1;
         freem[q] &= ~(1u << k);
         const int sl = kSentSlots * q + k;
-        slot_put(S, sl, in_row[q], g);
+        pend[q] = sl;
         if (lane == 0) { sq[y] = (short)sl; s.srank[sl] = tq[y]; }
       }
     }
