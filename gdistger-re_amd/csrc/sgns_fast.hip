@@ -861,11 +861,16 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       wave_sync();
       typedef short __attribute__((ext_vector_type(2))) short2v;
       __attribute__((address_space(1))) short2v* const wb = (__attribute__((address_space(1))) short2v*)(a.woh);
-      for (int i = 0; i < os; i++) {
-        const unsigned v = *reinterpret_cast<const unsigned*>(mdl + i * MDS + 4 * lane);
-        const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (D / 2);
-        (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + lane, __builtin_bit_cast(short2v, v));
-      }
+      // all rows are read first (one wait), then one atomic per output row
+      unsigned v[kOutRows];
+#pragma unroll
+      for (int i = 0; i < kOutRows; i++) v[i] = *reinterpret_cast<const unsigned*>(mdl + i * MDS + 4 * lane);
+#pragma unroll
+      for (int i = 0; i < kOutRows; i++)
+        if (i < os) {
+          const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (D / 2);
+          (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + lane, __builtin_bit_cast(short2v, v[i]));
+        }
     }
 
     // Wih rows entering the windows at the transition below; they take their
