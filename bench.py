@@ -45,7 +45,8 @@ HBM_PEAK_GBS = 8000.0
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
 METRIC = "trained words/sec, BF16-MFMA vs FP32 plain, at \u22640.005 link-pred AUC gap"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_train_rmat21_traffic.json")
+# PMC traffic summary of a trainer kernel on the R-MAT 2^21 workload
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "{kernel}_rmat21_traffic.json")
 AUC_GAP_MAX = 0.005
 
 
@@ -201,6 +202,7 @@ def main() -> int:
             "tflops": h["contraction_tflops"],
             "peak": peak_tf,
             "frac": h["contraction_tflops"] / peak_tf,
+            "executed_mfma": executed_mfma(workload, head, kernel, h["trained_words_per_step"], h["kernel_ms_avg"]),
         },
     }
     if "fp32" in results and head != "fp32":
@@ -231,14 +233,34 @@ def measured_traffic(workload, form, kernel, trained_words):
     bytes per trained word times this launch's trained words; None if the
     summary does not cover this workload and form."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILE.format(kernel=kernel)) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
     e = t.get("forms", {}).get(form)
-    if t.get("workload") != workload or not e or e.get("kernel", "k_train") != kernel:
+    if t.get("workload") != workload or not e or e.get("kernel") != kernel:
         return None
     return e["hbm_bytes_per_word"] * trained_words
+
+
+def executed_mfma(workload, form, kernel, trained_words, kernel_ms):
+    """MFMA work the hardware executed (padding included), from the committed
+    PMC summary of the kernel: SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 flop per
+    step times this launch's steps (two trained words per step), over this
+    launch's kernel time; None if the summary does not cover this workload."""
+    try:
+        with open(os.path.join(ROOT, "profiles", f"{kernel}_rmat21_counters.json")) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("form") != form or kernel_ms <= 0:
+        return None
+    ps = t["per_step"]
+    flop = ps["SQ_INSTS_VALU_MFMA_MOPS_BF16"] * 512.0 * trained_words / 2.0
+    tf = flop / (kernel_ms * 1e-3) / 1e12
+    return {"tflops": tf, "frac": tf / BF16_MFMA_PEAK_TFLOPS,
+            "mfma_busy_cycles_per_step": ps["SQ_VALU_MFMA_BUSY_CYCLES"],
+            "source": f"profiles/{kernel}_rmat21_counters.json"}
 
 
 def host_cpus():
