@@ -47,6 +47,7 @@
 // lane (g, m) reads rows 4g .. 4g+3, elements 8m .. 8m+7.  The next step's
 // rows are read right after this step's updates (so they include them).
 
+#include <mutex>
 #include <hip/hip_runtime.h>
 
 #include "dger_sgns.h"
@@ -1082,6 +1083,38 @@ hipError_t launch_step_fast(int os, const int32_t* meta, const int32_t* mult, fl
                             const float* O, const float* I, float* corr, float* md, float* I_out, hipStream_t st) {
   hipLaunchKernelGGL(k_step_fast, dim3(1), dim3(64), 0, st, os, meta, mult, alpha, exp_table, O, I, corr, md, I_out);
   return hipGetLastError();
+}
+
+// ---- scratch of the allocation-free entry point ------------------------------------
+// One file-scope device array: each GPU the library runs on holds its own copy
+// (kNoAllocScratchBytes) until the process ends.  In host code its name is a
+// stand-in, not a device address: the address comes from hipGetSymbolAddress,
+// taken once per device with that device current and kept here.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, void** ptr) {
+  *ptr = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;  // the caller makes the device current
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return hipSuccess;
 }
 
 }  // namespace dger

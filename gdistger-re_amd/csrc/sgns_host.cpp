@@ -369,6 +369,12 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   c->fast = opts->precision == DGER_PRECISION_BF16_MFMA &&
             dger::fast_supported(opts->size, opts->negative, opts->window, v_num, opts->table_size);
   auto bail = [&](int code) { delete c; return code; };
+  // The allocation-free entry point's scratch address on this device is
+  // looked up once here (dger_ctx_train_noalloc reports a failure).
+  if (c->fast) {
+    void* sp = nullptr;
+    (void)dger::noalloc_scratch(opts->device, &sp);
+  }
   if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess)
     return bail(fail(DGER_E_HIP, "stream/event creation failed"));
@@ -614,12 +620,10 @@ static int autotune_workers(dger_ctx* c, int cus, int wpb_max, int* out) {
   return DGER_OK;
 }
 
-int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, uint64_t* worker_steps,
-                   dger_sgns_stats* stats) {
-  if (!c) return fail(DGER_E_ARG, "null ctx");
-  if (!c->have_corpus) return fail(DGER_E_STATE, "dger_ctx_set_corpus must come first");
-  if (rec && rec_cap <= 0) return fail(DGER_E_ARG, "rec_cap must be positive");
-  DGER_HIP(hipSetDevice(c->o.device));
+// Worker count and workers per workgroup of a training run: `workers` if
+// positive, else the options' count, the tuned count, the autotuner (when
+// allowed) or the default.
+static int pick_workers(dger_ctx* c, int32_t workers, bool allow_autotune, int* out_workers, int* out_wpb) {
   hipDeviceProp_t prop;
   DGER_HIP(hipGetDeviceProperties(&prop, c->o.device));
   // Workgroups may take more LDS than the default per-block figure by opting in.
@@ -639,7 +643,7 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
   const int wpb_max = wpb;
   if (workers <= 0) workers = c->o.workers;
   if (workers <= 0 && c->tuned_workers > 0) workers = c->tuned_workers;
-  if (workers <= 0 && c->o.autotune && !rec) {
+  if (workers <= 0 && c->o.autotune && allow_autotune) {
     const int rc = autotune_workers(c, prop.multiProcessorCount, wpb_max, &workers);
     if (rc != DGER_OK) return rc;
     c->tuned_workers = workers;
@@ -652,6 +656,41 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
             workers, wpb,
             block_lds(wpb),
             (size_t)prop.sharedMemPerBlock, (size_t)prop.sharedMemPerBlockOptin);
+  *out_workers = workers;
+  *out_wpb = wpb;
+  return DGER_OK;
+}
+
+// Run statistics from the device counters and the first worker's final rate.
+static int read_stats(dger_ctx* c, const dger::TrainArgs& a, float ms, dger_sgns_stats* stats) {
+  unsigned long long cnt[dger::kNumCounters];
+  DGER_HIP(hipMemcpy(cnt, a.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  float alpha0 = 0.f;
+  DGER_HIP(hipMemcpy(&alpha0, a.worker_alpha, 4, hipMemcpyDeviceToHost));
+  std::memset(stats, 0, sizeof(*stats));
+  stats->steps = cnt[dger::kSteps];
+  stats->trained_words = 2 * cnt[dger::kSteps];
+  stats->raw_words = cnt[dger::kRawWords];
+  stats->sentences = cnt[dger::kSentences];
+  stats->contexts = cnt[dger::kContexts];
+  stats->out_ctx = cnt[dger::kOutCtx];
+  stats->final_alpha = alpha0;
+  stats->workers = a.workers;
+  stats->vocab_size = (int32_t)c->V;
+  stats->max_sentence = c->Lmax;
+  stats->kernel_ms = ms;
+  stats->words_per_sec = ms > 0 ? (double)stats->trained_words / (ms * 1e-3) : 0.0;
+  return DGER_OK;
+}
+
+int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, uint64_t* worker_steps,
+                   dger_sgns_stats* stats) {
+  if (!c) return fail(DGER_E_ARG, "null ctx");
+  if (!c->have_corpus) return fail(DGER_E_STATE, "dger_ctx_set_corpus must come first");
+  if (rec && rec_cap <= 0) return fail(DGER_E_ARG, "rec_cap must be positive");
+  DGER_HIP(hipSetDevice(c->o.device));
+  int wpb = 0;
+  if (const int rc = pick_workers(c, workers, !rec, &workers, &wpb); rc != DGER_OK) return rc;
   const int stride = dger::record_stride(c->o.window, c->o.negative);
   if (const int rc = ensure_worker_buffers(c, workers); rc != DGER_OK) return rc;
   if (rec && c->rec_d.ensure((size_t)workers * (size_t)rec_cap * stride * 4))
@@ -662,31 +701,61 @@ int dger_ctx_train(dger_ctx* c, int32_t workers, int32_t* rec, int64_t rec_cap, 
   float ms = 0.f;
   if (const int rc = run_pass(c, a, c->o.graph != 0 && !rec, &ms); rc != DGER_OK) return rc;
 
-  unsigned long long cnt[dger::kNumCounters];
-  DGER_HIP(hipMemcpy(cnt, c->counters_d.p, sizeof(cnt), hipMemcpyDeviceToHost));
-  std::vector<unsigned long long> ws(workers);
-  DGER_HIP(hipMemcpy(ws.data(), c->wsteps_d.p, (size_t)workers * 8, hipMemcpyDeviceToHost));
-  float alpha0 = 0.f;
-  DGER_HIP(hipMemcpy(&alpha0, c->walpha_d.p, 4, hipMemcpyDeviceToHost));
   if (worker_steps)
-    for (int w = 0; w < workers; w++) worker_steps[w] = ws[w];
+    DGER_HIP(hipMemcpy(worker_steps, c->wsteps_d.p, (size_t)workers * 8, hipMemcpyDeviceToHost));
   if (rec)
     DGER_HIP(hipMemcpy(rec, c->rec_d.p, (size_t)workers * (size_t)rec_cap * stride * 4, hipMemcpyDeviceToHost));
-  if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
-    stats->steps = cnt[dger::kSteps];
-    stats->trained_words = 2 * cnt[dger::kSteps];
-    stats->raw_words = cnt[dger::kRawWords];
-    stats->sentences = cnt[dger::kSentences];
-    stats->contexts = cnt[dger::kContexts];
-    stats->out_ctx = cnt[dger::kOutCtx];
-    stats->final_alpha = alpha0;
-    stats->workers = workers;
-    stats->vocab_size = (int32_t)c->V;
-    stats->max_sentence = c->Lmax;
-    stats->kernel_ms = ms;
-    stats->words_per_sec = ms > 0 ? (double)stats->trained_words / (ms * 1e-3) : 0.0;
+  if (stats)
+    if (const int rc = read_stats(c, a, ms, stats); rc != DGER_OK) return rc;
+  return DGER_OK;
+}
+
+static size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Offsets in the allocation-free entry point's scratch: counters, per-worker
+// steps, per-worker final rates, per-worker negative batches; returns the
+// bytes needed.
+static size_t noalloc_layout(dger_ctx* c, int workers, size_t* off) {
+  off[0] = 0;
+  off[1] = al256((size_t)dger::kNumCounters * 8);
+  off[2] = off[1] + al256((size_t)workers * 8);
+  off[3] = off[2] + al256((size_t)workers * 4);
+  return off[3] + (size_t)workers * dger::fast_scratch_ints(c->Lmax, c->o.negative) * 4;
+}
+
+int dger_ctx_train_noalloc(dger_ctx* c, int32_t workers, dger_sgns_stats* stats) {
+  if (!c) return fail(DGER_E_ARG, "null ctx");
+  if (!c->have_corpus) return fail(DGER_E_STATE, "dger_ctx_set_corpus must come first");
+  if (!c->fast)
+    return fail(DGER_E_UNSUPPORTED,
+                "dger_ctx_train_noalloc runs the window-slot trainer only (DGER_PRECISION_BF16_MFMA, "
+                "size 128, window <= 11, negative <= 14); use dger_ctx_train");
+  DGER_HIP(hipSetDevice(c->o.device));
+  void* sp = nullptr;
+  DGER_HIP(dger::noalloc_scratch(c->o.device, &sp));
+  int wpb = 0;
+  if (const int rc = pick_workers(c, workers, false, &workers, &wpb); rc != DGER_OK) return rc;
+  size_t off[4];
+  const size_t need = noalloc_layout(c, workers, off);
+  if (need > dger::kNoAllocScratchBytes) {
+    int fit = 0;
+    for (int w = 1; w <= workers; w *= 2)
+      if (noalloc_layout(c, w, off) <= dger::kNoAllocScratchBytes) fit = w;
+    return fail(DGER_E_ARG, std::to_string(workers) + " workers need " + std::to_string(need) +
+                                " bytes of scratch; dger_ctx_train_noalloc holds " +
+                                std::to_string(dger::kNoAllocScratchBytes) + " (at most " + std::to_string(fit) +
+                                " workers fit at this sentence length and negative count)");
   }
+  unsigned char* const base = static_cast<unsigned char*>(sp);
+  dger::TrainArgs a = make_args(c, workers, wpb, nullptr, 0, c->n_tokens);
+  a.counters = reinterpret_cast<unsigned long long*>(base + off[0]);
+  a.worker_steps = reinterpret_cast<unsigned long long*>(base + off[1]);
+  a.worker_alpha = reinterpret_cast<float*>(base + off[2]);
+  a.fscratch = reinterpret_cast<float*>(base + off[3]);
+  float ms = 0.f;
+  if (const int rc = run_pass(c, a, false, &ms); rc != DGER_OK) return rc;
+  if (stats)
+    if (const int rc = read_stats(c, a, ms, stats); rc != DGER_OK) return rc;
   return DGER_OK;
 }
 

@@ -90,6 +90,11 @@ DGER_HD inline size_t fast_scratch_ints(int Lmax, int K) { return (size_t)(K > 0
 size_t fast_block_lds_bytes(int Lmax, int K, int wpb);
 int fast_workers_per_block(int Lmax, int K, size_t lds_per_block);
 hipError_t prepare_train_fast(const TrainArgs& a);
+// Device scratch of the allocation-free trainer entry point: a static device
+// array of kNoAllocScratchBytes per GPU.  noalloc_scratch returns this
+// device's address of it (looked up once per device; `device` must be current).
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+hipError_t noalloc_scratch(int device, void** ptr);
 hipError_t launch_train_fast(const TrainArgs& a, hipStream_t s);
 // One step of the window-slot form on host-staged rows (test-facing).
 hipError_t launch_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha, const float* exp_table,

@@ -102,6 +102,7 @@ _SIGS = {
     "dger_ctx_set_corpus": (C.c_int, [_vp, _vp, C.c_uint64]),
     "dger_ctx_reset_model": (C.c_int, [_vp]),
     "dger_ctx_train": (C.c_int, [_vp, _i32, _vp, C.c_int64, _vp, C.POINTER(Stats)]),
+    "dger_ctx_train_noalloc": (C.c_int, [_vp, _i32, C.POINTER(Stats)]),
     "dger_ctx_get_model": (C.c_int, [_vp, _vp, _vp]),
     "dger_ctx_set_model": (C.c_int, [_vp, _vp, _vp]),
     "dger_ctx_get_vocab": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
@@ -338,6 +339,13 @@ class Trainer:
             raise ValueError("records need an explicit worker count")
         _check(lib().dger_ctx_train(self._h, workers, _ptr(rec), rec_cap, _ptr(ws), C.byref(st)))
         return st, rec, ws
+
+    def train_noalloc(self, workers: int = 0) -> "Stats":
+        """dger_ctx_train_noalloc: one run with no device allocation (the
+        window-slot trainer, scratch in the library's static device array)."""
+        st = Stats()
+        _check(lib().dger_ctx_train_noalloc(self._h, workers, C.byref(st)))
+        return st
 
     def link_scores(self, pairs) -> np.ndarray:
         """Dot-product scores of vertex-id pairs on the device-resident model."""

@@ -158,6 +158,18 @@ int dger_ctx_reset_model(dger_ctx *ctx);
  * `workers` entries) receives each worker's step count. */
 int dger_ctx_train(dger_ctx *ctx, int32_t workers, int32_t *rec, int64_t rec_cap,
                    uint64_t *worker_steps, dger_sgns_stats *stats);
+/* dger_ctx_train without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate, such as code under HIP graph
+ * capture, tight serving loops or harnesses that forbid hipMalloc.  Runs the
+ * window-slot trainer (DGER_PRECISION_BF16_MFMA contexts it supports; other
+ * contexts: DGER_E_UNSUPPORTED) with its per-run scratch -- counters, per-worker
+ * step counts and rates, per-worker negative batches -- in a static device
+ * array of 16 MiB per GPU (held until the process ends), so that
+ * 4 * (negative * max_sentence + 4) bytes per worker plus 2 KiB must fit:
+ * larger requests fail with DGER_E_ARG and name the worker count that fits.
+ * No autotuning, records or graph replay; stats as dger_ctx_train.  Every call
+ * on one GPU uses that GPU's array: calls that overlap on one GPU race. */
+int dger_ctx_train_noalloc(dger_ctx *ctx, int32_t workers, dger_sgns_stats *stats);
 /* Copies Wih / Woh (V x size float32, rank order) to the host; either may be NULL. */
 int dger_ctx_get_model(dger_ctx *ctx, float *wih, float *woh);
 /* Replaces Wih / Woh from the host; either may be NULL. */
