@@ -951,6 +951,28 @@ int dger_sgns_step(int32_t D, int32_t n_out, int32_t n_ctx, int32_t n_ctx_first,
   return DGER_OK;
 }
 
+int dger_sgns_step_mfma_gemm1(int32_t D, int32_t n_out, int32_t n_ctx, int32_t n_ctx_first, const int32_t* meta,
+                              float alpha, const float* O, const float* I, float* corr) {
+  if (D < 32 || D > 256 || D % 32 || n_out < 2 || n_out > 8 || n_ctx < 1 || n_ctx > 64 || n_ctx_first < 0 ||
+      n_ctx_first > n_ctx || !meta || !O || !I || !corr)
+    return fail(DGER_E_ARG, "bad MFMA step shape (size a multiple of 32 up to 256, 2..8 outputs, 1..64 contexts)");
+  float exp_table[dger::kExpTable];
+  build_exp_table(exp_table);
+  DevBuf dm, de, dO, dI, dc;
+  if (dm.ensure(n_out * 4) || de.ensure(sizeof(exp_table)) || dO.ensure((size_t)n_out * D * 4) ||
+      dI.ensure((size_t)n_ctx * D * 4) || dc.ensure((size_t)n_out * n_ctx * 4))
+    return fail(DGER_E_ALLOC, "device allocation failed");
+  DGER_HIP(hipMemcpy(dm.p, meta, n_out * 4, hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(de.p, exp_table, sizeof(exp_table), hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(dO.p, O, (size_t)n_out * D * 4, hipMemcpyHostToDevice));
+  DGER_HIP(hipMemcpy(dI.p, I, (size_t)n_ctx * D * 4, hipMemcpyHostToDevice));
+  DGER_HIP(dger::launch_step_mfma_gemm1(D, n_out, n_ctx, n_ctx_first, dm.as<int32_t>(), alpha, de.as<float>(),
+                                        dO.as<float>(), dI.as<float>(), dc.as<float>(), nullptr));
+  DGER_HIP(hipDeviceSynchronize());
+  DGER_HIP(hipMemcpy(corr, dc.p, (size_t)n_out * n_ctx * 4, hipMemcpyDeviceToHost));
+  return DGER_OK;
+}
+
 int dger_sgns_step_fast(int32_t n_out, const int32_t* meta, const int32_t* mult, float alpha, const float* O,
                         const float* I, float* corr, float* md, float* I_out) {
   const int D = dger::kFastDim, S = 48;

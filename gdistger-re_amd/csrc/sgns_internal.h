@@ -135,6 +135,11 @@ hipError_t launch_link_auc(const float* scores, const int8_t* labels, uint64_t n
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s);
 hipError_t launch_bf16_to_f32(const uint16_t* src, float* dst, uint64_t n, hipStream_t s);
+// GEMM1 + sigmoid of the BF16-MFMA form on given rows (test-facing): D a
+// multiple of 32 up to 256, 2 <= os <= 8, 1 <= nin <= 64.
+hipError_t launch_step_mfma_gemm1(int D, int os, int nin, int fin, const int32_t* meta, float alpha,
+                                  const float* exp_table, const float* O, const float* I, float* corr,
+                                  hipStream_t s);
 hipError_t launch_step(int D, int os, int nin, int fin, const int32_t* meta, float alpha,
                        const float* exp_table, const float* O, const float* I, float* corr,
                        float* md, float* gin, hipStream_t s);

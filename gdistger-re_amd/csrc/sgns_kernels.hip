@@ -1714,6 +1714,31 @@ __global__ __launch_bounds__(64) void k_step(int D, int os, int nin, int fin, co
   }
 }
 
+// GEMM1 + sigmoid of the BF16-MFMA step (mfma_gemm1, the trainer's code for
+// steps of at most kOC outputs) on given rows: corr[i][j] = the coefficient
+// of output i and context j.
+template <int NP>
+__global__ __launch_bounds__(64) void k_step_mfma_gemm1(int D, int os, int nin, int fin, const int32_t* meta,
+                                                        float alpha, const float* exp_table, const float* O,
+                                                        const float* I, float* corr) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x;
+  const int K = os - 2, W = (nin + 3) / 4 > 0 ? (nin + 3) / 4 : 1;
+  const int OST = o_stride(D), RST = ring_stride(D);
+  const Lds s = carve(smem, 1, K, W, D);
+  for (int i = lane; i < ko8(K); i += 64) plan_meta(s, K, 0)[i] = i < os ? meta[i] : 0;
+  for (int i = 0; i < ko8(K); i++)
+    for (int d = lane; d < OST; d += 64) s.O[i * OST + d] = (i < os && d < D) ? O[(size_t)i * D + d] : 0.f;
+  for (int j = 0; j < nin; j++)
+    for (int d = lane; d < D; d += 64) s.ring[(size_t)j * RST + d] = I[(size_t)j * D + d];
+  for (int j = lane; j < nin; j += 64) s.ckey[j] = j;
+  __syncthreads();
+  mfma_gemm1<NP>(s, s.ring, RST, plan_meta(s, K, 0), D, os, nin, fin, alpha, exp_table, lane);
+  __syncthreads();
+  for (int i = 0; i < os; i++)
+    for (int j = lane; j < nin; j += 64) corr[(size_t)i * nin + j] = s.Ct[(size_t)j * cmf_stride(K) + i];
+}
+
 // ---- SGNS logistic loss and adjoint (exact sigmoid) -------------------------
 template <typename T>
 __device__ __forceinline__ T softplus(T x) {
@@ -1888,6 +1913,23 @@ hipError_t launch_train(const TrainArgs& a, int precision, hipStream_t s) {
   if (precision == DGER_PRECISION_BF16 || precision == DGER_PRECISION_BF16_MFMA)
     return launch_train_t<uint16_t>(a, s, precision);
   return launch_train_t<float>(a, s, precision);
+}
+
+hipError_t launch_step_mfma_gemm1(int D, int os, int nin, int fin, const int32_t* meta, float alpha,
+                                  const float* exp_table, const float* O, const float* I, float* corr,
+                                  hipStream_t s) {
+  if (D % 32 != 0 || D < 32 || D > 256 || os < 2 || os > kOC || nin < 1 || nin > 64 || fin < 0 || fin > nin)
+    return hipErrorInvalidValue;
+  const int K = os - 2, W = (nin + 3) / 4;
+  size_t off[7];
+  const size_t lds = lds_layout(1, K, W, D, off);
+  if (np_class(D) == 1)
+    hipLaunchKernelGGL((k_step_mfma_gemm1<1>), dim3(1), dim3(64), lds, s, D, os, nin, fin, meta, alpha, exp_table,
+                       O, I, corr);
+  else
+    hipLaunchKernelGGL((k_step_mfma_gemm1<2>), dim3(1), dim3(64), lds, s, D, os, nin, fin, meta, alpha, exp_table,
+                       O, I, corr);
+  return hipGetLastError();
 }
 
 hipError_t launch_step(int D, int os, int nin, int fin, const int32_t* meta, float alpha,

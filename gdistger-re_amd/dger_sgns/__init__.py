@@ -103,6 +103,7 @@ _SIGS = {
     "dger_ctx_reset_model": (C.c_int, [_vp]),
     "dger_ctx_train": (C.c_int, [_vp, _i32, _vp, C.c_int64, _vp, C.POINTER(Stats)]),
     "dger_ctx_train_noalloc": (C.c_int, [_vp, _i32, C.POINTER(Stats)]),
+    "dger_sgns_step_mfma_gemm1": (C.c_int, [_i32, _i32, _i32, _i32, _vp, C.c_float, _vp, _vp, _vp]),
     "dger_ctx_get_model": (C.c_int, [_vp, _vp, _vp]),
     "dger_ctx_set_model": (C.c_int, [_vp, _vp, _vp]),
     "dger_ctx_get_vocab": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
@@ -426,6 +427,19 @@ def step(O, I, fin: int, meta, alpha: float):
     _check(lib().dger_sgns_step(D, os_, nin, fin, _ptr(meta), alpha, _ptr(O), _ptr(I), _ptr(corr),
                                 _ptr(md), _ptr(gin)))
     return corr, md, gin
+
+
+def step_mfma_gemm1(O, I, fin: int, meta, alpha: float) -> np.ndarray:
+    """GEMM1 + sigmoid gradient of the BF16-MFMA general trainer on the device
+    (dger_sgns_step_mfma_gemm1): returns corr (n_out x n_ctx)."""
+    O = np.ascontiguousarray(O, np.float32)
+    I = np.ascontiguousarray(I, np.float32)
+    meta = _i32arr(meta)
+    os_, D = O.shape
+    nin = I.shape[0]
+    corr = np.empty((os_, nin), np.float32)
+    _check(lib().dger_sgns_step_mfma_gemm1(D, os_, nin, fin, _ptr(meta), alpha, _ptr(O), _ptr(I), _ptr(corr)))
+    return corr
 
 
 FAST_DIM = 128

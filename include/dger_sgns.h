@@ -196,6 +196,17 @@ int dger_sgns_step(int32_t D, int32_t n_out, int32_t n_ctx, int32_t n_ctx_first,
                    const int32_t *meta, float alpha, const float *O, const float *I,
                    float *corr, float *md, float *gin);
 
+/* GEMM1 and the sigmoid gradient (dsgl.hpp:1241-1272) of the BF16-MFMA form's
+ * general trainer -- the code its minibatches of at most 8 outputs and 64
+ * contexts run (sizes other than the window-slot trainer's) -- on
+ * host-staged rows: bf16 operands, FP32 accumulation.  D a multiple of 32 up
+ * to 256, n_out in [2, 8], n_ctx in [1, 64]; contexts [0, n_ctx_first) label
+ * row 0, the rest row n_out-1.  corr (n_out x n_ctx) receives the
+ * coefficients meta * (label - sigmoid) * alpha. */
+int dger_sgns_step_mfma_gemm1(int32_t D, int32_t n_out, int32_t n_ctx, int32_t n_ctx_first,
+                              const int32_t *meta, float alpha, const float *O, const float *I,
+                              float *corr);
+
 /* One step of the BF16-MFMA form's window-slot trainer on host-staged rows
  * (the same device code as that trainer; see DGER_PRECISION_BF16_MFMA).
  * Row length 128.  O (n_out x 128, n_out in [2, 16]) holds the step's output
