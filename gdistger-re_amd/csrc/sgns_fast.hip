@@ -72,8 +72,10 @@ constexpr int kSentSlots = 24;        // slots of one sentence
 constexpr int kOutRows = 16;          // output rows per step (MFMA tile)
 constexpr int RB = 2 * D;             // bytes of a bf16 row in LDS
 constexpr int CBS = 144;              // bytes per output row of the transposed coefficients
-constexpr int MDS = 2 * D + 16;       // bytes per output row of the staged Woh updates
+constexpr int MDS = 4 * D + 16;       // bytes per output row of the staged Woh updates (FP32)
 constexpr int kMdOff = kOutRows * CBS; // staged Woh updates: in rc, after the coefficients
+constexpr int kStageOff = 4096;        // FP32 master output rows of the next step, in rc (past a new
+                                       // sentence's first-window rows, which the loader stages at its start)
 constexpr int kNone = 1 << 30;
 constexpr short kNone16 = 0x7FFF;     // "no next occurrence" in the 16-bit position arrays
 constexpr int kExpOne = kExpTable, kExpZero = kExpTable + 1;  // sigmoid 1 and 0 after the table
@@ -238,9 +240,7 @@ struct Slots {
 
 // Slot s (wave-uniform) takes the row whose elements 8m .. 8m+7 this lane
 // holds in v; only the owner group's lanes change their registers.
-__device__ __forceinline__ void slot_put(Slots& S, int s, const u32x4& v, int g) {
-  float e[8];
-  unpack8(v, e);
+__device__ __forceinline__ void slot_put_f(Slots& S, int s, const float (&e)[8], int g) {
   const bool own = g == ((s >> 2) & 3);
   switch (((s >> 4) << 2) | (s & 3)) {
 #define DGER_PUT(RT, RR)                                             \
@@ -253,6 +253,13 @@ __device__ __forceinline__ void slot_put(Slots& S, int s, const u32x4& v, int g)
 #undef DGER_PUT
     default: break;
   }
+}
+
+// The same from a bf16 row (elements 8m .. 8m+7 in v).
+__device__ __forceinline__ void slot_put(Slots& S, int s, const u32x4& v, int g) {
+  float e[8];
+  unpack8(v, e);
+  slot_put_f(S, s, e, g);
 }
 
 // Elements 8m .. 8m+7 of slot s held by this lane (meaningful in the owner group).
@@ -287,6 +294,14 @@ __device__ __forceinline__ u32x4 pack8_sr(const float (&e)[8], unsigned seed) {
     w[k] = __builtin_bit_cast(unsigned, v);
     h ^= h << 13; h ^= h >> 17; h ^= h << 5;
   }
+  return w;
+}
+
+// Eight floats to bf16, round to nearest even.
+__device__ __forceinline__ u32x4 pack8_rne(const float (&e)[8]) {
+  u32x4 w;
+#pragma unroll
+  for (int k = 0; k < 4; k++) w[k] = pack_rne(e[2 * k], e[2 * k + 1]);
   return w;
 }
 
@@ -451,6 +466,9 @@ __host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15
 
 static_assert(kOutRows * CBS <= kSlots * RB, "the coefficients live inside the slot copy");
 static_assert(kMdOff + kOutRows * MDS <= kSlots * RB, "the staged updates live inside the slot copy");
+static_assert(kStageOff >= kSentSlots / 2 * RB && kStageOff + kOutRows * 4 * D <= kSlots * RB,
+              "the staged master rows live inside the slot copy, past the first-window rows");
+static_assert(2 * 4 * D <= kOutRows * RB, "two entering master rows fit in the output rows' space");
 
 __host__ __device__ inline size_t fast_layout(int Lmax, int K, size_t* off) {
   (void)K;
@@ -494,17 +512,66 @@ __device__ inline void load_exp_table(float* expt, const float* src, int tid, in
 
 // Output rows of a plan for GEMM3's B operand (rows 4g .. 4g+3, elements
 // 8m .. 8m+7); rows past os read row 0 of the plan (their results are not
-// used).
-__device__ __forceinline__ void load_output_l3(const Mat& woh, const int* plan, int os, int g, int m, u32x4 (&o3)[4]) {
+// used).  A row with an FP32 master (rank < nm) is copied instead into the
+// LDS staging area (row i at stage + i * 4D bytes, straight from memory), and
+// out_rows_from_stage fills it in once the loads have been waited for.
+__device__ __forceinline__ void load_output_l3(const Mat& woh, const Mat& om, int nm, const int* plan, int os, int g,
+                                               int m, int lane, u32x4 (&o3)[4], unsigned char* stage) {
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     const int i = 4 * g + e;
     const int r3 = plan[i < os ? i : 0];
-    o3[e] = woh.load((unsigned)r3 * (D * 2) + 16 * m);
+    if (nm == 0 || r3 >= nm) o3[e] = woh.load((unsigned)r3 * (D * 2) + 16 * m);
+  }
+  for (int i = 0; nm > 0 && i < os; i++) {
+    const int rk = __builtin_amdgcn_readfirstlane(plan[i]);
+    if (rk < nm) {
+      om.load_lds((unsigned)rk * (D * 4) + 4 * lane, stage + i * (D * 4));
+      om.load_lds((unsigned)rk * (D * 4) + 256 + 4 * lane, stage + i * (D * 4) + 256);
+    }
   }
 }
+__device__ __forceinline__ void out_rows_from_stage(u32x4 (&o3)[4], int nm, const int* plan, int os, int g, int m,
+                                                    const unsigned char* stage) {
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const int i = 4 * g + e;
+    const int src = i < os ? i : 0;
+    if (plan[src] < nm) {
+      const f32x4 lo = *reinterpret_cast<const f32x4*>(stage + src * (D * 4) + 32 * m);
+      const f32x4 hi = *reinterpret_cast<const f32x4*>(stage + src * (D * 4) + 32 * m + 16);
+      o3[e] = (u32x4){pack_rne(lo[0], lo[1]), pack_rne(lo[2], lo[3]), pack_rne(hi[0], hi[1]), pack_rne(hi[2], hi[3])};
+    }
+  }
+}
+// Elements 8m .. 8m+7 of FP32 master row rk.
+__device__ __forceinline__ void master_get(const Mat& M, int rk, int m, float (&e)[8]) {
+  const u32x4 lo = M.load((unsigned)rk * (D * 4) + 32 * m), hi = M.load((unsigned)rk * (D * 4) + 32 * m + 16);
+#pragma unroll
+  for (int k = 0; k < 4; k++) { e[k] = __uint_as_float(lo[k]); e[4 + k] = __uint_as_float(hi[k]); }
+}
+__device__ __forceinline__ void master_put(const Mat& M, int rk, int m, const float (&e)[8]) {
+  u32x4 lo, hi;
+#pragma unroll
+  for (int k = 0; k < 4; k++) { lo[k] = __float_as_uint(e[k]); hi[k] = __float_as_uint(e[4 + k]); }
+  M.store((unsigned)rk * (D * 4) + 32 * m, lo);
+  M.store((unsigned)rk * (D * 4) + 32 * m + 16, hi);
+}
 
-template <int WPB>
+// A window row goes back to Wih: its FP32 master (if it has one) exactly, or the
+// bf16 row with stochastic (or nearest) rounding.
+__device__ __forceinline__ void write_back(const Mat& wih, const Mat& wm, bool master, int rk, int m,
+                                           const float (&e)[8], int round_nearest, unsigned seed) {
+  if (master)
+    master_put(wm, rk, m, e);
+  else
+    wih.store((unsigned)rk * (D * 2) + 16 * m, round_nearest ? pack8_rne(e) : pack8_sr(e, seed));
+}
+
+// kLadder: the instance with FP32 master rows and stochastic rounding of the
+// Woh updates available (a.n_master, a.round_nearest); the other instance
+// has neither and none of their code.
+template <int WPB, bool kLadder>
 __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & 63;
@@ -521,6 +588,14 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   size_t off_unused[6];
   const FastLds s = fast_carve(smem + exp_bytes() + (size_t)wave * fast_layout(Lmax, K, off_unused), Lmax, K);
   const Mat wih(a.wih, (unsigned)((size_t)a.V * D * 2)), woh(a.woh, (unsigned)((size_t)a.V * D * 2));
+  // FP32 master rows of the hottest ranks (rank < nmaster).
+  const int nmaster = kLadder ? a.n_master : 0;
+  const int round_nearest = kLadder ? a.round_nearest : 1;
+  auto is_master = [&](int rk) { return kLadder && rk < nmaster; };
+  // Neither masters nor stochastic rounding of the Woh updates: the bf16
+  // paths below take their shorter forms.
+  const bool plain = !kLadder || (nmaster == 0 && round_nearest);
+  const Mat wm(a.wmaster, (unsigned)((size_t)nmaster * D * 4)), om(a.omaster, (unsigned)((size_t)nmaster * D * 4));
   // The shared negative batch of the worker: K x Lmax draws in device memory.
   int* const negw = reinterpret_cast<int*>(a.fscratch) + (size_t)wk * fast_scratch_ints(Lmax, K);
 
@@ -567,8 +642,12 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   bool planned = false;
   int pre_os = 0;
   u32x4 o3[4];              // output rows of the next step (GEMM3 layout), on their way to ost
-  u32x4 in_row[2];          // rows entering the windows at a transition
+  u32x4 in_row[2];          // bf16 rows entering the windows at a transition
   int pend[2] = {-1, -1};   // their slots, filled at the next step's start
+  bool pend_hot[2] = {false, false};  // the row comes from an FP32 master (staged in ost)
+  int in_rk[2] = {0, 0};    // ranks of those rows
+  bool staged = false;      // FP32 master rows are on their way into LDS
+  unsigned char* const ostage = s.rc + kStageOff;  // FP32 master output rows of the next step
   unsigned salt = hash32((unsigned)wk * 0x9E3779B9u + 0x6A09E667u);
 
   // Learning-rate decay, sentence loads, slice end, refresh: as the reference
@@ -645,10 +724,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         // Slots of the first window [0, W]: the k-th distinct rank takes slot 24q + k.
         const int nw = min(W, L - 1) + 1;
         const bool isnew = lane < nw && s.prv[q * Lmax + lane] < 0;
-        const u64 nm = __ballot(isnew);
-        const int nnew = __popcll(nm);
+        const u64 newmask = __ballot(isnew);
+        const int nnew = __popcll(newmask);
         int myslot = 0;
-        if (lane < nw) myslot = kSentSlots * q + __popcll(nm & lanemask_lt(sq[lane]));
+        if (lane < nw) myslot = kSentSlots * q + __popcll(newmask & lanemask_lt(sq[lane]));
         wave_sync();
         if (lane < nw) sq[lane] = (short)myslot;
         if (isnew) s.srank[myslot] = tq[lane];
@@ -662,8 +741,16 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
           wih.load_lds((unsigned)rk * (D * 2) + 4 * lane, s.rc + k * (D * 2));
         }
         wait_vm_all();
-        for (int k = 0; k < nnew; k++)
-          slot_put(S, kSentSlots * q + k, *reinterpret_cast<const u32x4*>(s.rc + k * (D * 2) + 16 * m), g);
+        for (int k = 0; k < nnew; k++) {
+          const int rk = __builtin_amdgcn_readfirstlane(s.srank[kSentSlots * q + k]);
+          if (is_master(rk)) {
+            float e[8];
+            master_get(wm, rk, m, e);
+            slot_put_f(S, kSentSlots * q + k, e, g);
+          } else {
+            slot_put(S, kSentSlots * q + k, *reinterpret_cast<const u32x4*>(s.rc + k * (D * 2) + 16 * m), g);
+          }
+        }
         wave_sync();
         planned = false;
       }
@@ -795,7 +882,11 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       os = pre_os;
     } else {
       os = make_outputs(plan, t1, t2, (lane < K) ? negw[grp * K + lane] : -1);
-      load_output_l3(woh, s.plans + plan * 2 * kOutRows, os, g, m, o3);
+      load_output_l3(woh, om, nmaster, s.plans + plan * 2 * kOutRows, os, g, m, lane, o3, ostage);
+      if (nmaster > 0) {
+        wait_vm_all();
+        out_rows_from_stage(o3, nmaster, s.plans + plan * 2 * kOutRows, os, g, m, ostage);
+      }
     }
     const int* const cplan = s.plans + plan * 2 * kOutRows;
 
@@ -816,12 +907,24 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     // Rows that entered the windows at the last transition take their slots
     // now (their loads, issued during that step, are waited for here
     // together with this step's output rows).
+    if (staged) {
+      wait_vm_all();
+      staged = false;
+    }
 #pragma unroll
     for (int q = 0; q < 2; q++)
       if (pend[q] >= 0) {
-        slot_put(S, pend[q], in_row[q], g);
+        if (pend_hot[q]) {
+          const f32x4 lo = *reinterpret_cast<const f32x4*>(s.ost + q * (D * 4) + 32 * m);
+          const f32x4 hi = *reinterpret_cast<const f32x4*>(s.ost + q * (D * 4) + 32 * m + 16);
+          const float e[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+          slot_put_f(S, pend[q], e, g);
+        } else {
+          slot_put(S, pend[q], in_row[q], g);
+        }
         pend[q] = -1;
       }
+    if (planned && nmaster > 0) out_rows_from_stage(o3, nmaster, cplan, os, g, m, ostage);
     ring_copy(S, s.rc, g, m);
     // the output rows, bf16, for GEMM1's A operand and GEMM3's B operand
 #pragma unroll
@@ -837,10 +940,11 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     gemm1_coef(s.ost, L, meta4, os, alpha, expt, g, m, C);
     wave_sync();
     {
-      // md in bf16 pairs, staged row by row in LDS (rc is free until the
-      // next step's slot copy); then Woh row of output i += md row i with
-      // bf16 atomic adds, one row per instruction (lane l: elements 2l, 2l+1),
-      // so that each instruction covers whole 64-byte segments.
+      // md in FP32, staged row by row in LDS (rc is free until the next
+      // step's slot copy); then Woh row of output i += md row i, one row at a
+      // time: an FP32 master row takes FP32 atomic adds (lanes l and 64 + l),
+      // a bf16 row bf16 atomic adds of element pairs (lane l: elements 2l,
+      // 2l+1), each instruction covering whole 64-byte segments.
       unsigned char* const mdl = s.rc + kMdOff;
       const bf16x8 c0 = *reinterpret_cast<const bf16x8*>(s.cb + m * CBS + g * 32);
       const bf16x8 c1 = *reinterpret_cast<const bf16x8*>(s.cb + m * CBS + g * 32 + 16);
@@ -848,30 +952,71 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       for (int h = 0; h < 2; h++) {
         f32x4 md[NT / 2];
         gemm2_half(S, c0, c1, h * (NT / 2), md);
+        if (plain) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          u32x2 w;
-          w[0] = pack_rne(md[0][r], md[1][r]);
-          w[1] = pack_rne(md[2][r], md[3][r]);
-          *reinterpret_cast<u32x2*>(mdl + (4 * g + r) * MDS + 16 * m + 8 * h) = w;
+          for (int r = 0; r < 4; r++) {
+            u32x2 w;
+            w[0] = pack_rne(md[0][r], md[1][r]);
+            w[1] = pack_rne(md[2][r], md[3][r]);
+            *reinterpret_cast<u32x2*>(mdl + (4 * g + r) * MDS + 16 * m + 8 * h) = w;
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            *reinterpret_cast<f32x4*>(mdl + (4 * g + r) * MDS + 32 * m + 16 * h) =
+                (f32x4){md[0][r], md[1][r], md[2][r], md[3][r]};
         }
         // keep the halves apart (their registers would not fit together)
         __builtin_amdgcn_sched_barrier(0);
       }
       const int orank = (lane < kOutRows) ? cplan[lane] : 0;
+      // random bits of this step's stochastic roundings (rotated per row)
+      const unsigned rseed = plain ? 0u : hash32(salt ^ 0x85EBCA6Bu ^ ((unsigned)lane * 0x9E3779B9u));
       wave_sync();
       typedef short __attribute__((ext_vector_type(2))) short2v;
       __attribute__((address_space(1))) short2v* const wb = (__attribute__((address_space(1))) short2v*)(a.woh);
-      // all rows are read first (one wait), then one atomic per output row
-      unsigned v[kOutRows];
+      __attribute__((address_space(1))) float* const ob = (__attribute__((address_space(1))) float*)(a.omaster);
+      if (plain) {
+        // no masters, nearest rounding: bf16 pairs, all rows read first
+        unsigned v[kOutRows];
 #pragma unroll
-      for (int i = 0; i < kOutRows; i++) v[i] = *reinterpret_cast<const unsigned*>(mdl + i * MDS + 4 * lane);
+        for (int i = 0; i < kOutRows; i++) v[i] = *reinterpret_cast<const unsigned*>(mdl + i * MDS + 4 * lane);
 #pragma unroll
-      for (int i = 0; i < kOutRows; i++)
-        if (i < os) {
-          const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (D / 2);
-          (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + lane, __builtin_bit_cast(short2v, v[i]));
+        for (int i = 0; i < kOutRows; i++)
+          if (i < os) {
+            const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (D / 2);
+            (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + lane, __builtin_bit_cast(short2v, v[i]));
+          }
+      } else
+      for (int i = 0; i < os; i++) {
+        const int rk = __builtin_amdgcn_readlane(orank, i);
+        if (is_master(rk)) {
+          const float v0 = *reinterpret_cast<const float*>(mdl + i * MDS + 4 * lane);
+          const float v1 = *reinterpret_cast<const float*>(mdl + i * MDS + 256 + 4 * lane);
+          (void)__builtin_amdgcn_global_atomic_fadd_f32(ob + (size_t)rk * D + lane, v0);
+          (void)__builtin_amdgcn_global_atomic_fadd_f32(ob + (size_t)rk * D + 64 + lane, v1);
+        } else {
+          const float2 mv = *reinterpret_cast<const float2*>(mdl + i * MDS + 8 * lane);
+          unsigned u;
+          if (round_nearest) {
+            u = pack_rne(mv.x, mv.y);
+          } else {
+            // Stochastic rounding of (row value read for this step + md):
+            // the atomic adds the exact bf16 difference t - old, so that a
+            // row nobody else changed meanwhile ends at t, an unbiased
+            // rounding of old + md.
+            const unsigned oldp = *reinterpret_cast<const unsigned*>(s.ost + swz(i, lane >> 2) + 4 * (lane & 3));
+            const float o0 = __uint_as_float(oldp << 16), o1 = __uint_as_float(oldp & 0xFFFF0000u);
+            const unsigned rb = __builtin_amdgcn_alignbit(rseed, rseed, (unsigned)(7 * i) & 31u);
+            bf16x2 t = {};
+            t = __builtin_amdgcn_cvt_sr_bf16_f32(t, o0 + mv.x, rb, false);
+            t = __builtin_amdgcn_cvt_sr_bf16_f32(t, o1 + mv.y, rb << 16, true);
+            const unsigned tb = __builtin_bit_cast(unsigned, t);
+            u = pack_rne(__uint_as_float(tb << 16) - o0, __uint_as_float(tb & 0xFFFF0000u) - o1);
+          }
+          (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + (unsigned)rk * (D / 2) + lane, __builtin_bit_cast(short2v, u));
         }
+      }
     }
 
     // Wih rows entering the windows at the transition below; they take their
@@ -880,8 +1025,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int y = (q ? p1 : p0) + W + 1;
-      const int rk = (in_slot_src[q] == -1) ? __builtin_amdgcn_readfirstlane(s.tok[q * Lmax + y]) : 0;
-      in_row[q] = wih.load((unsigned)rk * (D * 2) + 16 * m);
+      const int rk = (in_slot_src[q] == -1) ? __builtin_amdgcn_readfirstlane(s.tok[q * Lmax + y]) : -1;
+      pend_hot[q] = rk >= 0 && is_master(rk);  // staged into ost after GEMM3 below
+      in_rk[q] = rk;
+      if (!pend_hot[q]) in_row[q] = wih.load((unsigned)(rk >= 0 ? rk : 0) * (D * 2) + 16 * m);
     }
     // Plan the next step's outputs and read its rows (after this step's
     // updates of the same rows), when neither sentence ends here.
@@ -890,7 +1037,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     if (cont) {
       const int np = plan ^ 1;
       osn = make_outputs(np, s.tok[p0 + 1], s.tok[Lmax + p1 + 1], vnext);
-      load_output_l3(woh, s.plans + np * 2 * kOutRows, osn, g, m, o3);
+      load_output_l3(woh, om, nmaster, s.plans + np * 2 * kOutRows, osn, g, m, lane, o3, ostage);
+      staged = nmaster > 0;
     } else {
       // the next step plans afresh: the rows are dead (and need no register
       // across this step)
@@ -898,6 +1046,14 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       for (int e = 0; e < 4; e++) o3[e] = (u32x4){0u, 0u, 0u, 0u};
     }
     gemm3(S, C, s.ost, g, m);
+    // entering rows with FP32 masters: into ost, which GEMM3 no longer needs
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+      if (pend_hot[q]) {
+        wm.load_lds((unsigned)in_rk[q] * (D * 4) + 4 * lane, s.ost + q * (D * 4));
+        wm.load_lds((unsigned)in_rk[q] * (D * 4) + 256 + 4 * lane, s.ost + q * (D * 4) + 256);
+        staged = true;
+      }
 
     steps++;
     if (lane == 0) {
@@ -921,7 +1077,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
           slot_get(S, sl, e);
           const int rk = __builtin_amdgcn_readfirstlane(s.srank[sl]);
           if (g == ((sl >> 2) & 3))
-            wih.store((unsigned)rk * (D * 2) + 16 * m, pack8_sr(e, salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sl << 24)));
+            write_back(wih, wm, is_master(rk), rk, m, e, round_nearest, salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sl << 24));
         }
         freem[q] = 0xFFFFFFu;
         len[q] = 0;
@@ -936,8 +1092,9 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         float e[8];
         slot_get(S, sx, e);
         if (g == ((sx >> 2) & 3))
-          wih.store((unsigned)__builtin_amdgcn_readfirstlane(tq[x]) * (D * 2) + 16 * m,
-                    pack8_sr(e, salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sx << 24)));
+          write_back(wih, wm, is_master(__builtin_amdgcn_readfirstlane(tq[x])), __builtin_amdgcn_readfirstlane(tq[x]), m, e,
+                     round_nearest,
+                     salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sx << 24));
         freem[q] |= 1u << (sx - kSentSlots * q);
       }
       if (in_slot_src[q] >= 0) {
@@ -1067,15 +1224,23 @@ int fast_workers_per_block(int Lmax, int K, size_t lds_per_block) {
 
 hipError_t prepare_train_fast(const TrainArgs& a) {
   if (a.wpb < 1 || a.wpb > kFastMaxWpb) return hipErrorInvalidValue;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train_fast<kFastMaxWpb>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)fast_block_lds_bytes(a.Lmax, a.K, a.wpb));
+  const int lds = (int)fast_block_lds_bytes(a.Lmax, a.K, a.wpb);
+  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train_fast<kFastMaxWpb, false>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train_fast<kFastMaxWpb, true>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 
 hipError_t launch_train_fast(const TrainArgs& a, hipStream_t st) {
   if (a.wpb < 1 || a.wpb > kFastMaxWpb) return hipErrorInvalidValue;
   const size_t lds = fast_block_lds_bytes(a.Lmax, a.K, a.wpb);
-  hipLaunchKernelGGL((k_train_fast<kFastMaxWpb>), dim3((a.workers + a.wpb - 1) / a.wpb), dim3(64 * a.wpb), lds, st, a);
+  if (a.n_master > 0 || !a.round_nearest)
+    hipLaunchKernelGGL((k_train_fast<kFastMaxWpb, true>), dim3((a.workers + a.wpb - 1) / a.wpb), dim3(64 * a.wpb), lds,
+                       st, a);
+  else
+    hipLaunchKernelGGL((k_train_fast<kFastMaxWpb, false>), dim3((a.workers + a.wpb - 1) / a.wpb), dim3(64 * a.wpb), lds,
+                       st, a);
   return hipGetLastError();
 }
 

@@ -201,6 +201,8 @@ int dger_sgns_default_opts(dger_sgns_opts* o) {
   o->device = 0;
   o->graph = 1;
   o->autotune = 0;
+  o->rounding = 1;
+  o->masters = 0;
   return DGER_OK;
 }
 
@@ -259,6 +261,13 @@ int dger_sgns_parse_args(int argc, char** argv, dger_sgns_opts* o) {
   DGER_INT("-device", device)
   DGER_INT("-graph", graph)
   DGER_INT("-autotune", autotune)
+  DGER_INT("-masters", masters)
+  if ((v = val("-rounding"))) {
+    if (!*v) return missing("-rounding");
+    if (!std::strcmp(v, "sr") || !std::strcmp(v, "0")) o->rounding = 0;
+    else if (!std::strcmp(v, "rne") || !std::strcmp(v, "1")) o->rounding = 1;
+    else return fail(DGER_E_ARG, std::string("unknown -rounding ") + v);
+  }
   if ((v = val("-precision"))) {
     if (!*v) return missing("-precision");
     if (!std::strcmp(v, "bf16") || !std::strcmp(v, "1")) o->precision = DGER_PRECISION_BF16;
@@ -304,6 +313,9 @@ static int store_model_from_tmp(dger_ctx* c, int m) {
     DGER_HIP(dger::launch_fp8_quantize(c->tmp_d.as<float>(), c->desc_d.as<dger::Fp8Mat>() + m, c->V, c->D, c->stream));
   } else if (c->o.precision != DGER_PRECISION_FP32) {
     DGER_HIP(dger::launch_f32_to_bf16(c->tmp_d.as<float>(), dst->as<uint16_t>(), n, c->stream));
+    if (c->fast && c->n_master > 0)  // the FP32 master rows of the top ranks
+      DGER_HIP(hipMemcpyAsync((m ? c->om_d : c->wm_d).p, c->tmp_d.p, (size_t)c->n_master * c->D * 4,
+                              hipMemcpyDeviceToDevice, c->stream));
   } else {
     DGER_HIP(hipMemcpyAsync(dst->p, c->tmp_d.p, n * 4, hipMemcpyDeviceToDevice, c->stream));
   }
@@ -319,6 +331,9 @@ static int load_model_to_tmp(dger_ctx* c, int m) {
     DGER_HIP(dger::launch_fp8_dequantize(c->desc_d.as<dger::Fp8Mat>() + m, c->tmp_d.as<float>(), c->V, c->D, c->stream));
   } else if (c->o.precision != DGER_PRECISION_FP32) {
     DGER_HIP(dger::launch_bf16_to_f32(src->as<uint16_t>(), c->tmp_d.as<float>(), n, c->stream));
+    if (c->fast && c->n_master > 0)  // the top ranks' rows are their FP32 masters
+      DGER_HIP(hipMemcpyAsync(c->tmp_d.p, (m ? c->om_d : c->wm_d).p, (size_t)c->n_master * c->D * 4,
+                              hipMemcpyDeviceToDevice, c->stream));
   } else {
     DGER_HIP(hipMemcpyAsync(c->tmp_d.p, src->p, n * 4, hipMemcpyDeviceToDevice, c->stream));
   }
@@ -341,6 +356,11 @@ int dger_ctx_reset_model(dger_ctx* c) {
     DGER_HIP(c->tmp_d.ensure(n * 4));
     DGER_HIP(dger::launch_init_net(c->tmp_d.as<float>(), n, c->D, c->stream));
     DGER_HIP(dger::launch_f32_to_bf16(c->tmp_d.as<float>(), c->wih_d.as<uint16_t>(), n, c->stream));
+    if (c->fast && c->n_master > 0) {
+      DGER_HIP(hipMemcpyAsync(c->wm_d.p, c->tmp_d.p, (size_t)c->n_master * c->D * 4, hipMemcpyDeviceToDevice,
+                              c->stream));
+      DGER_HIP(hipMemsetAsync(c->om_d.p, 0, (size_t)c->n_master * c->D * 4, c->stream));
+    }
   } else {
     DGER_HIP(dger::launch_init_net(c->wih_d.as<float>(), n, c->D, c->stream));
   }
@@ -423,6 +443,12 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
       hipMemcpyAsync(c->exp_d.p, exp_table, sizeof(exp_table), hipMemcpyHostToDevice, c->stream) ||
       hipMemcpyAsync(c->tmp_d.p, starts.data(), starts.size() * 8, hipMemcpyHostToDevice, c->stream))
     return bail(fail(DGER_E_HIP, "upload failed"));
+  if (c->fast && opts->masters) {
+    // Window-slot trainer: FP32 masters for the top 1 % ranks by count.
+    c->n_master = (int)std::max<uint64_t>(1, ((uint64_t)v_num + 99) / 100);
+    if (c->wm_d.ensure((size_t)c->n_master * c->D * 4) || c->om_d.ensure((size_t)c->n_master * c->D * 4))
+      return bail(fail(DGER_E_ALLOC, "device allocation failed"));
+  }
   if (opts->precision == DGER_PRECISION_FP8) {
     // FP32 masters for the top 1 % ranks by count (at least one row).
     c->n_master = (int)std::max<uint64_t>(1, ((uint64_t)v_num + 99) / 100);
@@ -536,6 +562,12 @@ static dger::TrainArgs make_args(dger_ctx* c, int workers, int wpb, int32_t* rec
   a.mfma = (c->o.precision == DGER_PRECISION_BF16_MFMA || fp8) ? 1 : 0;
   a.alpha0 = c->o.alpha;
   a.alpha_den = (float)((uint64_t)c->o.iter * c->train_words + 1);
+  if (c->fast) {
+    a.wmaster = c->wm_d.as<float>();
+    a.omaster = c->om_d.as<float>();
+    a.n_master = c->n_master;
+    a.round_nearest = c->o.rounding == 1 ? 1 : 0;
+  }
   return a;
 }
 
@@ -580,6 +612,14 @@ static int run_pass(dger_ctx* c, const dger::TrainArgs& a, bool use_graph, float
   DGER_HIP(hipEventRecord(c->ev1, c->stream));
   DGER_HIP(hipStreamSynchronize(c->stream));
   DGER_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  if (c->fast && c->n_master > 0) {
+    // The bf16 rows of the master ranks (read by link evaluation and the
+    // next pass only through their masters) follow their masters.
+    const uint64_t nmd = (uint64_t)c->n_master * c->D;
+    DGER_HIP(dger::launch_f32_to_bf16(c->wm_d.as<float>(), c->wih_d.as<uint16_t>(), nmd, c->stream));
+    DGER_HIP(dger::launch_f32_to_bf16(c->om_d.as<float>(), c->woh_d.as<uint16_t>(), nmd, c->stream));
+    DGER_HIP(hipStreamSynchronize(c->stream));
+  }
   return DGER_OK;
 }
 

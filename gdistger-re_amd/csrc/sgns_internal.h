@@ -57,6 +57,13 @@ struct TrainArgs {
   int32_t mfma;              // BF16 rows: run the contractions on MFMA
   float alpha0;
   float alpha_den;           // (float)(iter * train_words + 1)
+  // Window-slot trainer: FP32 master rows of the ranks below n_master (Wih,
+  // Woh; n_master x D floats each), and the rounding of bf16 row updates
+  // (0: stochastic, so that small updates survive in expectation; 1: nearest).
+  float* wmaster;
+  float* omaster;
+  int32_t n_master;
+  int32_t round_nearest;
 };
 
 // Floats of per-worker scratch: two sentence buffers (Lmax rows each) and the

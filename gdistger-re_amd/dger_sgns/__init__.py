@@ -55,6 +55,7 @@ class Opts(C.Structure):
         ("read_vocab", C.c_char * 512), ("save_emb", C.c_char * 512),
         ("workers", C.c_int32), ("precision", C.c_int32), ("table_size", C.c_int64),
         ("device", C.c_int32), ("graph", C.c_int32), ("autotune", C.c_int32),
+        ("rounding", C.c_int32), ("masters", C.c_int32),
     ]
 
 

@@ -87,6 +87,13 @@ typedef struct dger_sgns_opts {
                               once in a HIP graph and replayed by later passes (1) */
   int32_t autotune;        /* -autotune    1: with -workers 0, the first pass times candidate
                               worker counts on a corpus prefix and keeps the fastest (0) */
+  int32_t rounding;        /* -rounding    bf16 row stores and Woh updates of the window-slot
+                              trainer: sr (0) stochastic rounding, so that updates below half
+                              a bf16 step survive in expectation (multi-epoch or small-alpha
+                              runs, where nearest rounding stalls); rne (1) nearest (1) */
+  int32_t masters;         /* -masters     1: the window-slot trainer keeps FP32 master rows
+                              for the top 1 % ranks by count in Wih and Woh, read and
+                              updated in FP32 (0) */
 } dger_sgns_opts;
 
 typedef struct dger_sgns_stats {
