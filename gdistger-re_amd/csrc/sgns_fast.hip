@@ -568,10 +568,11 @@ __device__ __forceinline__ void write_back(const Mat& wih, const Mat& wm, bool m
     wih.store((unsigned)rk * (D * 2) + 16 * m, round_nearest ? pack8_rne(e) : pack8_sr(e, seed));
 }
 
-// kLadder: the instance with FP32 master rows and stochastic rounding of the
-// Woh updates available (a.n_master, a.round_nearest); the other instance
-// has neither and none of their code.
-template <int WPB, bool kLadder>
+// kFull: the instance with FP32 master rows, stochastic rounding of the Woh
+// updates (a.n_master, a.round_nearest) and minibatches smaller than a step
+// (a.minibatch) available; the other instance has none of them and none of
+// their code.  Both share negative groups over 2^a.share_shift steps.
+template <int WPB, bool kFull>
 __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & 63;
@@ -589,12 +590,16 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   const FastLds s = fast_carve(smem + exp_bytes() + (size_t)wave * fast_layout(Lmax, K, off_unused), Lmax, K);
   const Mat wih(a.wih, (unsigned)((size_t)a.V * D * 2)), woh(a.woh, (unsigned)((size_t)a.V * D * 2));
   // FP32 master rows of the hottest ranks (rank < nmaster).
-  const int nmaster = kLadder ? a.n_master : 0;
-  const int round_nearest = kLadder ? a.round_nearest : 1;
-  auto is_master = [&](int rk) { return kLadder && rk < nmaster; };
+  const int nmaster = kFull ? a.n_master : 0;
+  const int round_nearest = kFull ? a.round_nearest : 1;
+  auto is_master = [&](int rk) { return kFull && rk < nmaster; };
+  // Contexts per minibatch (0: a step's contexts at once) and negative
+  // sharing: steps n with equal n >> hs draw from one group.
+  const int mb = kFull ? a.minibatch : 0;
+  const int hs = a.share_shift;
   // Neither masters nor stochastic rounding of the Woh updates: the bf16
   // paths below take their shorter forms.
-  const bool plain = !kLadder || (nmaster == 0 && round_nearest);
+  const bool plain = !kFull || (nmaster == 0 && round_nearest);
   const Mat wm(a.wmaster, (unsigned)((size_t)nmaster * D * 4)), om(a.omaster, (unsigned)((size_t)nmaster * D * 4));
   // The shared negative batch of the worker: K x Lmax draws in device memory.
   int* const negw = reinterpret_cast<int*>(a.fscratch) + (size_t)wk * fast_scratch_ints(Lmax, K);
@@ -649,6 +654,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   bool staged = false;      // FP32 master rows are on their way into LDS
   unsigned char* const ostage = s.rc + kStageOff;  // FP32 master output rows of the next step
   unsigned salt = hash32((unsigned)wk * 0x9E3779B9u + 0x6A09E667u);
+  int j0 = 0;      // first context of the step's current minibatch
+  int b_step = 0;  // the step's window shrink, drawn at its first minibatch
 
   // Learning-rate decay, sentence loads, slice end, refresh: as the reference
   // loop (dsgl.hpp:886-1088).
@@ -769,9 +776,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       continue;
     }
 
-    // Shared negative batch (dsgl.hpp:1054-1080): K * max(L1, L2) draws.
+    // Shared negative batch (dsgl.hpp:1054-1080): K * max(L1, L2) draws, or
+    // K per 2^hs steps of the longer sentence.
     if (refresh) {
-      nbn = max(len[0], len[1]);
+      nbn = (max(len[0], len[1]) + (1 << hs) - 1) >> hs;
       divN = make_div32((unsigned)nbn);
       const int NN = nbn * K;
       const unsigned ts = (unsigned)a.table_size, vm1 = (unsigned)(a.V - 1);
@@ -814,14 +822,24 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     const int p0 = pos[0], p1 = pos[1];
     const int t1 = __builtin_amdgcn_readfirstlane(s.tok[p0]);
     const int t2 = __builtin_amdgcn_readfirstlane(s.tok[Lmax + p1]);
-    rng = rng * kLcgMul + kLcgAdd;
-    const int b = (int)mod64_small(rng, divW);
+    int b;
+    if (!kFull || j0 == 0) {
+      rng = rng * kLcgMul + kLcgAdd;
+      b = (int)mod64_small(rng, divW);
+      b_step = b;
+    } else {
+      b = b_step;
+    }
     const int hw = W - b;
     const int lo0 = max(0, p0 - hw), hi0 = min(len[0] - 1, p0 + hw);
     const int lo1 = max(0, p1 - hw), hi1 = min(len[1] - 1, p1 + hw);
     const int n0 = (p0 - lo0) + (hi0 - p0);
     const int n1 = (p1 - lo1) + (hi1 - p1);
     const int nin = n0 + n1, fin = n0;
+    // This minibatch: contexts [jb, je) of the step's list.
+    const int jb = (kFull && mb > 0) ? j0 : 0;
+    const int je = (kFull && mb > 0) ? min(nin, j0 + mb) : nin;
+    const bool last = je >= nin;
 
     // Positions entering the windows at this step's transition.
     int in_slot_src[2];  // position whose slot the entering position shares, or -1 (new row)
@@ -845,7 +863,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       const int jj = q ? lane - n0 : lane;
       const int lo = q ? lo1 : lo0, pp = q ? p1 : p0;
       const int c = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
-      atomicAdd(&s.mult[s.sop[q * Lmax + c]], 1);
+      if (lane >= jb && lane < je) atomicAdd(&s.mult[s.sop[q * Lmax + c]], 1);
       my_ctx_rank = s.tok[q * Lmax + c];
     }
 
@@ -853,10 +871,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     // group with multiplicities (a negative equal to the target is dropped),
     // second target; rows past the count have multiplicity 0.
     // v: negative k2 of the group in lane k2 < K.
-    neg_count++;
-    const int grp = (int)mod32((unsigned)neg_count, divN);
+    if (!kFull || j0 == 0) neg_count++;
+    const int grp = (int)mod32((unsigned)neg_count >> hs, divN);
     // The next step's group, read now for the planning after GEMM2.
-    const int vnext = (lane < K) ? negw[(int)mod32((unsigned)(neg_count + 1), divN) * K + lane] : -1;
+    const int vnext = (lane < K) ? negw[(int)mod32((unsigned)(neg_count + 1) >> hs, divN) * K + lane] : -1;
     auto make_outputs = [&](int pl, int ta, int tb, int v) -> int {
       int* const pr = s.plans + pl * 2 * kOutRows;
       const bool live = lane < K;
@@ -890,7 +908,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     }
     const int* const cplan = s.plans + plan * 2 * kOutRows;
 
-    if (rec && (int64_t)steps < a.rec_cap) {
+    if (rec && (int64_t)steps < a.rec_cap && (!kFull || j0 == 0)) {
       int32_t* r = rec + (size_t)steps * stride;
       for (int z = lane; z < stride; z += 64) r[z] = -1;
       wave_sync();
@@ -1025,14 +1043,14 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int y = (q ? p1 : p0) + W + 1;
-      const int rk = (in_slot_src[q] == -1) ? __builtin_amdgcn_readfirstlane(s.tok[q * Lmax + y]) : -1;
+      const int rk = (last && in_slot_src[q] == -1) ? __builtin_amdgcn_readfirstlane(s.tok[q * Lmax + y]) : -1;
       pend_hot[q] = rk >= 0 && is_master(rk);  // staged into ost after GEMM3 below
       in_rk[q] = rk;
       if (!pend_hot[q]) in_row[q] = wih.load((unsigned)(rk >= 0 ? rk : 0) * (D * 2) + 16 * m);
     }
     // Plan the next step's outputs and read its rows (after this step's
     // updates of the same rows), when neither sentence ends here.
-    const bool cont = (p0 + 1 < len[0]) && (p1 + 1 < len[1]);
+    const bool cont = last && (p0 + 1 < len[0]) && (p1 + 1 < len[1]);
     int osn = 0;
     if (cont) {
       const int np = plan ^ 1;
@@ -1055,11 +1073,21 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         staged = true;
       }
 
-    steps++;
     if (lane == 0) {
-      stat_add(s.stats, kStatCtx, (u64)nin);
-      stat_add(s.stats, kStatOutCtx, (u64)(os * nin));
+      stat_add(s.stats, kStatCtx, (u64)(je - jb));
+      stat_add(s.stats, kStatOutCtx, (u64)(os * (je - jb)));
     }
+    if (kFull && !last) {
+      // The next minibatch of this step: its output rows are read again
+      // (after the updates above); the windows stay.
+      j0 = je;
+      planned = false;
+      salt = hash32(salt + 0x9E3779B9u);
+      wave_sync();
+      continue;
+    }
+    j0 = 0;
+    steps++;
 
     // Transition: a finished sentence writes its live slots back; a
     // continuing one slides its window by one position.
@@ -1235,7 +1263,7 @@ hipError_t prepare_train_fast(const TrainArgs& a) {
 hipError_t launch_train_fast(const TrainArgs& a, hipStream_t st) {
   if (a.wpb < 1 || a.wpb > kFastMaxWpb) return hipErrorInvalidValue;
   const size_t lds = fast_block_lds_bytes(a.Lmax, a.K, a.wpb);
-  if (a.n_master > 0 || !a.round_nearest)
+  if (a.n_master > 0 || !a.round_nearest || a.minibatch > 0)
     hipLaunchKernelGGL((k_train_fast<kFastMaxWpb, true>), dim3((a.workers + a.wpb - 1) / a.wpb), dim3(64 * a.wpb), lds,
                        st, a);
   else

@@ -132,7 +132,8 @@ int dger::host_fail(int code, const std::string& msg) { return fail(code, msg); 
 
 struct dger_ctx {
   dger_sgns_opts o;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // where device work is queued
+  hipStream_t own_stream = nullptr;  // the context's own stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint32_t V = 0;
   int D = 0;
@@ -148,6 +149,9 @@ struct dger_ctx {
   hipGraphExec_t graph_exec = nullptr;
   dger::TrainArgs graph_args{};
   int tuned_workers = 0;  // autotuner's choice (0 = not tuned)
+  // Negative sharing and minibatch size of the runs (the options', or the
+  // autotuner's choice under -autotune 2).
+  int share = 2, minibatch = 0;
   DevBuf link_pairs_d, link_scores_d, link_labels_d, link_scratch_d, link_counts_d;
   // FP8 form: master rows of Wih / Woh and the two matrix descriptors.
   DevBuf wm_d, om_d, desc_d;
@@ -166,7 +170,7 @@ struct dger_ctx {
     drop_graph();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 };
 
@@ -203,6 +207,8 @@ int dger_sgns_default_opts(dger_sgns_opts* o) {
   o->autotune = 0;
   o->rounding = 1;
   o->masters = 0;
+  o->batch_parity = 1;
+  o->share = 2;
   return DGER_OK;
 }
 
@@ -262,6 +268,8 @@ int dger_sgns_parse_args(int argc, char** argv, dger_sgns_opts* o) {
   DGER_INT("-graph", graph)
   DGER_INT("-autotune", autotune)
   DGER_INT("-masters", masters)
+  DGER_INT("-batch-parity", batch_parity)
+  DGER_INT("-share", share)
   if ((v = val("-rounding"))) {
     if (!*v) return missing("-rounding");
     if (!std::strcmp(v, "sr") || !std::strcmp(v, "0")) o->rounding = 0;
@@ -301,7 +309,21 @@ static int check_opts(const dger_sgns_opts* o) {
   if (o->precision == DGER_PRECISION_FP8 && (o->size % 4 != 0 || o->window > 16 || o->negative > 6))
     return fail(DGER_E_UNSUPPORTED, "-precision fp8 needs -size divisible by 4, -window <= 16, -negative <= 6");
   if (o->workers < 0) return fail(DGER_E_ARG, "-workers must be >= 0");
+  if (o->share != 2 && o->share != 4 && o->share != 8 && o->share != 16)
+    return fail(DGER_E_ARG, "-share must be 2, 4, 8 or 16");
+  if (o->batch_parity != 0 && o->batch_parity != 1) return fail(DGER_E_ARG, "-batch-parity must be 0 or 1");
+  if (!o->batch_parity && o->batch_size < 1) return fail(DGER_E_ARG, "-batch-size must be >= 1");
+  if (o->autotune < 0 || o->autotune > 2) return fail(DGER_E_ARG, "-autotune must be 0, 1 or 2");
   return DGER_OK;
+}
+
+// Minibatch size a run trains with: 0 (a step's contexts at once) under the
+// parity switch, where the reference replaces -batch-size by 2*window+1
+// (dsgl.hpp:832) and never splits a step (dsgl.hpp:1163-1167); else
+// -batch-size, or 0 when no step can exceed it (2 windows of 2*window).
+static int run_minibatch(const dger_sgns_opts& o) {
+  if (o.batch_parity) return 0;
+  return o.batch_size >= 4 * o.window ? 0 : o.batch_size;
 }
 
 // The model in the context's storage form from FP32 rows (rank order) in
@@ -389,15 +411,23 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   c->fast = opts->precision == DGER_PRECISION_BF16_MFMA &&
             dger::fast_supported(opts->size, opts->negative, opts->window, v_num, opts->table_size);
   auto bail = [&](int code) { delete c; return code; };
+  if (opts->batch_parity) c->o.batch_size = 2 * opts->window + 1;  // dsgl.hpp:832
+  c->share = opts->share;
+  c->minibatch = run_minibatch(c->o);
+  if ((c->share != 2 || c->minibatch != 0 || opts->autotune == 2) && !c->fast)
+    return bail(fail(DGER_E_UNSUPPORTED,
+                     "-share, -batch-parity 0 and -autotune 2 need the window-slot trainer "
+                     "(-precision bf16-mfma, -size 128, -window <= 11, -negative <= 14)"));
   // The allocation-free entry point's scratch address on this device is
   // looked up once here (dger_ctx_train_noalloc reports a failure).
   if (c->fast) {
     void* sp = nullptr;
     (void)dger::noalloc_scratch(opts->device, &sp);
   }
-  if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+  if (hipStreamCreate(&c->own_stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess)
     return bail(fail(DGER_E_HIP, "stream/event creation failed"));
+  c->stream = c->own_stream;
 
   // Vocabulary (dsgl.hpp:314-382): ranks by count, std::sort with the
   // reference's comparator on (cn, id) records in id order.
@@ -468,6 +498,20 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   rc = dger_ctx_reset_model(c);
   if (rc) return bail(rc);
   *out = c;
+  return DGER_OK;
+}
+
+int dger_ctx_set_stream(dger_ctx* c, void* stream) {
+  if (!c) return fail(DGER_E_ARG, "null context");
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+  if (stream) {
+    hipDevice_t dev = -1;
+    DGER_HIP(hipStreamGetDevice(s, &dev));
+    if ((int)dev != c->o.device) return fail(DGER_E_ARG, "stream belongs to another device");
+  }
+  DGER_HIP(hipSetDevice(c->o.device));
+  DGER_HIP(hipStreamSynchronize(c->stream));
+  c->stream = s;
   return DGER_OK;
 }
 
@@ -567,6 +611,8 @@ static dger::TrainArgs make_args(dger_ctx* c, int workers, int wpb, int32_t* rec
     a.omaster = c->om_d.as<float>();
     a.n_master = c->n_master;
     a.round_nearest = c->o.rounding == 1 ? 1 : 0;
+    a.share_shift = c->share == 16 ? 3 : c->share == 8 ? 2 : c->share == 4 ? 1 : 0;
+    a.minibatch = c->minibatch;
   }
   return a;
 }
@@ -623,10 +669,14 @@ static int run_pass(dger_ctx* c, const dger::TrainArgs& a, bool use_graph, float
   return DGER_OK;
 }
 
-// Worker-count autotuner: times one pass over a corpus prefix for one to four
-// workers per CU (eight for the window-slot trainer; one workgroup per CU) and
-// returns the fastest in trained words per second.  The model is saved before and restored after.
-static int autotune_workers(dger_ctx* c, int cus, int wpb_max, int* out) {
+// Autotuner: times one pass over a corpus prefix per candidate and keeps the
+// fastest in trained words per second.  Level 1: one to four workers per CU
+// (eight for the window-slot trainer), one workgroup per CU.  Level 2 (the
+// window-slot trainer) then also times, at that worker count, negative
+// sharing 2, 4, 8 with and without -batch-size 2*window+1 minibatches; these
+// change the training, so the caller gates the choice on embedding quality.
+// The model is saved before and restored after.
+static int autotune_run(dger_ctx* c, int cus, int wpb_max, int* out) {
   DevBuf* model[4] = {&c->wih_d, &c->woh_d, &c->wm_d, &c->om_d};
   DevBuf save[4];
   for (int m = 0; m < 4; m++) {
@@ -635,22 +685,49 @@ static int autotune_workers(dger_ctx* c, int cus, int wpb_max, int* out) {
     DGER_HIP(hipMemcpyAsync(save[m].p, model[m]->p, model[m]->bytes, hipMemcpyDeviceToDevice, c->stream));
   }
   const uint64_t sample = std::min<uint64_t>(c->n_tokens, 16ull << 20);
-  double best = -1.0;
-  int best_w = 0;
-  for (int per_cu = 1; per_cu <= std::min(c->fast ? dger::kFastMaxWpb : 4, wpb_max); per_cu++) {
-    const int w = std::max<int>(1, std::min<uint64_t>((uint64_t)cus * per_cu, std::max<uint64_t>(1, sample / 4096)));
+  auto time_pass = [&](int w, int wpb, double* rate) -> int {
     if (const int rc = ensure_worker_buffers(c, w); rc != DGER_OK) return rc;
-    const dger::TrainArgs a = make_args(c, w, std::min(per_cu, w), nullptr, 0, sample);
+    const dger::TrainArgs a = make_args(c, w, wpb, nullptr, 0, sample);
     float ms = 0.f;
     if (const int rc = run_pass(c, a, false, &ms); rc != DGER_OK) return rc;
     unsigned long long steps = 0;
     DGER_HIP(hipMemcpy(&steps, c->counters_d.as<unsigned long long>() + dger::kSteps, 8, hipMemcpyDeviceToHost));
-    const double rate = ms > 0 ? 2.0 * (double)steps / ms : 0.0;
+    *rate = ms > 0 ? 2.0 * (double)steps / ms : 0.0;
+    return DGER_OK;
+  };
+  double best = -1.0;
+  int best_w = 0, best_wpb = 1;
+  for (int per_cu = 1; per_cu <= std::min(c->fast ? dger::kFastMaxWpb : 4, wpb_max); per_cu++) {
+    const int w = std::max<int>(1, std::min<uint64_t>((uint64_t)cus * per_cu, std::max<uint64_t>(1, sample / 4096)));
+    double rate = 0.0;
+    if (const int rc = time_pass(w, std::min(per_cu, w), &rate); rc != DGER_OK) return rc;
     if (c->o.debug > 2) fprintf(stderr, "dger: autotune %d workers: %.1f M words/s\n", w, rate * 1e-3);
     if (rate > best) {
       best = rate;
       best_w = w;
+      best_wpb = std::min(per_cu, w);
     }
+  }
+  if (c->o.autotune >= 2 && c->fast) {
+    const int share0 = c->share, mb0 = c->minibatch;  // timed above
+    int best_share = share0, best_mb = mb0;
+    for (int share = 2; share <= 8; share *= 2)
+      for (const int mb : {0, 2 * c->o.window + 1}) {
+        if (share == share0 && mb == mb0) continue;
+        c->share = share;
+        c->minibatch = mb;
+        double rate = 0.0;
+        if (const int rc = time_pass(best_w, best_wpb, &rate); rc != DGER_OK) return rc;
+        if (c->o.debug > 2)
+          fprintf(stderr, "dger: autotune share %d minibatch %d: %.1f M words/s\n", share, mb, rate * 1e-3);
+        if (rate > best) {
+          best = rate;
+          best_share = share;
+          best_mb = mb;
+        }
+      }
+    c->share = best_share;
+    c->minibatch = best_mb;
   }
   for (int m = 0; m < 4; m++)
     if (model[m]->p)
@@ -684,7 +761,7 @@ static int pick_workers(dger_ctx* c, int32_t workers, bool allow_autotune, int* 
   if (workers <= 0) workers = c->o.workers;
   if (workers <= 0 && c->tuned_workers > 0) workers = c->tuned_workers;
   if (workers <= 0 && c->o.autotune && allow_autotune) {
-    const int rc = autotune_workers(c, prop.multiProcessorCount, wpb_max, &workers);
+    const int rc = autotune_run(c, prop.multiProcessorCount, wpb_max, &workers);
     if (rc != DGER_OK) return rc;
     c->tuned_workers = workers;
   }
@@ -718,6 +795,8 @@ static int read_stats(dger_ctx* c, const dger::TrainArgs& a, float ms, dger_sgns
   stats->workers = a.workers;
   stats->vocab_size = (int32_t)c->V;
   stats->max_sentence = c->Lmax;
+  stats->share = c->share;
+  stats->minibatch = c->minibatch;
   stats->kernel_ms = ms;
   stats->words_per_sec = ms > 0 ? (double)stats->trained_words / (ms * 1e-3) : 0.0;
   return DGER_OK;
@@ -906,23 +985,9 @@ int dger_ctx_get_table(dger_ctx* c, int32_t* table) {
 int dger_ctx_save(dger_ctx* c, const char* path, int32_t binary) {
   if (!c || !path) return fail(DGER_E_ARG, "null argument");
   std::vector<float> wih((size_t)c->V * c->D);
-  int rc = dger_ctx_get_model(c, wih.data(), nullptr);
+  const int rc = dger_ctx_get_model(c, wih.data(), nullptr);
   if (rc) return rc;
-  FILE* fo = std::fopen(path, "wb");
-  if (!fo) return fail(DGER_E_IO, std::string("cannot open ") + path);
-  std::fprintf(fo, "%ld %ld\n", (long)c->V, (long)c->D);
-  for (uint32_t a = 0; a < c->V; a++) {
-    std::fprintf(fo, "%u ", (unsigned)c->rank_id[a]);
-    const float* row = &wih[(size_t)a * c->D];
-    if (binary) {
-      std::fwrite(row, sizeof(float), (size_t)c->D, fo);
-    } else {
-      for (int b = 0; b < c->D; b++) std::fprintf(fo, "%f ", row[b]);
-    }
-    std::fprintf(fo, "\n");
-  }
-  if (std::fclose(fo) != 0) return fail(DGER_E_IO, std::string("write failed: ") + path);
-  return DGER_OK;
+  return dger_emb_write(path, c->V, c->D, c->rank_id.data(), wih.data(), binary);
 }
 
 void dger_ctx_destroy(dger_ctx* c) {
@@ -934,10 +999,17 @@ void dger_ctx_destroy(dger_ctx* c) {
 int dger_sgns_train(const dger_sgns_opts* opts, const int32_t* corpus, uint64_t n_tokens,
                     const int32_t* vertex_cn, uint32_t v_num, float* emb_out, int32_t* rank_id_out,
                     dger_sgns_stats* stats) {
+  return dger_sgns_train_stream(opts, corpus, n_tokens, vertex_cn, v_num, emb_out, rank_id_out, stats, nullptr);
+}
+
+int dger_sgns_train_stream(const dger_sgns_opts* opts, const int32_t* corpus, uint64_t n_tokens,
+                           const int32_t* vertex_cn, uint32_t v_num, float* emb_out, int32_t* rank_id_out,
+                           dger_sgns_stats* stats, void* stream) {
   dger_ctx* c = nullptr;
   int rc = dger_ctx_create(opts, vertex_cn, v_num, &c);
   if (rc) return rc;
-  rc = dger_ctx_set_corpus(c, corpus, n_tokens);
+  rc = dger_ctx_set_stream(c, stream);
+  if (!rc) rc = dger_ctx_set_corpus(c, corpus, n_tokens);
   if (!rc) rc = dger_ctx_train(c, 0, nullptr, 0, nullptr, stats);
   if (!rc && emb_out) rc = dger_ctx_get_model(c, emb_out, nullptr);
   if (!rc && rank_id_out) rc = dger_ctx_get_vocab(c, rank_id_out, nullptr);

@@ -64,6 +64,11 @@ struct TrainArgs {
   float* omaster;
   int32_t n_master;
   int32_t round_nearest;
+  // Window-slot trainer batching: 2^share_shift consecutive steps draw from
+  // one negative group (negative sharing 2 << share_shift); minibatch > 0
+  // splits a step's contexts into minibatches of at most that many.
+  int32_t share_shift;
+  int32_t minibatch;
 };
 
 // Floats of per-worker scratch: two sentence buffers (Lmax rows each) and the

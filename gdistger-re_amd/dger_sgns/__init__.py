@@ -55,7 +55,8 @@ class Opts(C.Structure):
         ("read_vocab", C.c_char * 512), ("save_emb", C.c_char * 512),
         ("workers", C.c_int32), ("precision", C.c_int32), ("table_size", C.c_int64),
         ("device", C.c_int32), ("graph", C.c_int32), ("autotune", C.c_int32),
-        ("rounding", C.c_int32), ("masters", C.c_int32),
+        ("rounding", C.c_int32), ("masters", C.c_int32), ("batch_parity", C.c_int32),
+        ("share", C.c_int32),
     ]
 
 
@@ -65,6 +66,7 @@ class Stats(C.Structure):
         ("sentences", C.c_uint64), ("final_alpha", C.c_float), ("workers", C.c_int32),
         ("vocab_size", C.c_int32), ("max_sentence", C.c_int32), ("kernel_ms", C.c_double),
         ("words_per_sec", C.c_double), ("contexts", C.c_uint64), ("out_ctx", C.c_uint64),
+        ("share", C.c_int32), ("minibatch", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -98,6 +100,9 @@ _SIGS = {
     "dger_sgns_parse_args": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(Opts)]),
     "dger_sgns_train": (C.c_int, [C.POINTER(Opts), _vp, C.c_uint64, _vp, C.c_uint32, _vp, _vp,
                                   C.POINTER(Stats)]),
+    "dger_sgns_train_stream": (C.c_int, [C.POINTER(Opts), _vp, C.c_uint64, _vp, C.c_uint32, _vp, _vp,
+                                         C.POINTER(Stats), _vp]),
+    "dger_ctx_set_stream": (C.c_int, [_vp, _vp]),
     "dger_sgns_main": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), _vp, C.c_uint64, _vp, C.c_uint32]),
     "dger_ctx_create": (C.c_int, [C.POINTER(Opts), _vp, C.c_uint32, C.POINTER(_vp)]),
     "dger_ctx_set_corpus": (C.c_int, [_vp, _vp, C.c_uint64]),
@@ -110,6 +115,7 @@ _SIGS = {
     "dger_ctx_get_vocab": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
     "dger_ctx_get_table": (C.c_int, [_vp, _vp]),
     "dger_ctx_save": (C.c_int, [_vp, C.c_char_p, _i32]),
+    "dger_emb_write": (C.c_int, [C.c_char_p, C.c_uint32, _i32, _vp, _vp, _i32]),
     "dger_ctx_destroy": (None, [_vp]),
     "dger_sgns_step": (C.c_int, [_i32, _i32, _i32, _i32, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp]),
     "dger_sgns_step_fast": (C.c_int, [_i32, _vp, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp]),
@@ -323,6 +329,12 @@ class Trainer:
         """Corpus from walks left in HBM (dger_ctx_set_corpus_walks)."""
         _check(lib().dger_ctx_set_corpus_walks(self._h, walks._h))
 
+    def set_stream(self, stream=None) -> None:
+        """Queue the context's device work on a caller's HIP stream (an
+        integer handle such as torch.cuda.Stream().cuda_stream; None = the
+        context's own stream)."""
+        _check(lib().dger_ctx_set_stream(self._h, stream))
+
     def reset_model(self) -> None:
         _check(lib().dger_ctx_reset_model(self._h))
 
@@ -392,15 +404,62 @@ class Trainer:
         _check(lib().dger_ctx_save(self._h, path.encode(), binary))
 
 
-def train(opts: Opts, corpus, vertex_cn):
-    """dger_sgns_train: returns (embeddings V x D in rank order, rank_id, stats)."""
+def emb_write(path: str, ids, rows, binary: int = 0) -> None:
+    """dger_emb_write: rows (V x D float32) with their vertex ids in the
+    reference's embedding file format (text or binary)."""
+    r = np.ascontiguousarray(rows, np.float32)
+    i = _i32arr(ids)
+    if r.ndim != 2 or i.size != r.shape[0]:
+        raise ValueError("one id per row")
+    _check(lib().dger_emb_write(path.encode(), r.shape[0], r.shape[1], _ptr(i), _ptr(r), binary))
+
+
+def emb_read(path: str, binary: int = 0):
+    """Reads an embedding file written with the given -binary: returns
+    (ids int64 [V], rows float32 [V, D])."""
+    with open(path, "rb") as f:
+        data = f.read()
+    nl = data.index(b"\n")
+    V, D = (int(x) for x in data[:nl].split())
+    ids = np.empty(V, np.int64)
+    rows = np.empty((V, D), np.float32)
+    p = nl + 1
+    for r in range(V):
+        sp = data.index(b" ", p)
+        ids[r] = int(data[p:sp])
+        p = sp + 1
+        if binary:
+            rows[r] = np.frombuffer(data, np.float32, D, p)
+            p += 4 * D
+        else:
+            nxt = data.index(b"\n", p)
+            vals = data[p:nxt].split()
+            if len(vals) != D:
+                raise ValueError(f"row {r}: {len(vals)} values")
+            rows[r] = np.array(vals).astype(np.float32)
+            p = nxt
+        if data[p:p + 1] != b"\n":
+            raise ValueError(f"row {r}: no newline")
+        p += 1
+    if p != len(data):
+        raise ValueError("trailing bytes")
+    return ids, rows
+
+
+def train(opts: Opts, corpus, vertex_cn, stream=None):
+    """dger_sgns_train (dger_sgns_train_stream when a HIP stream handle is
+    given): returns (embeddings V x D in rank order, rank_id, stats)."""
     c = _i32arr(corpus)
     cn = _i32arr(vertex_cn)
     emb = np.empty((cn.size, opts.size), np.float32)
     rank_id = np.empty(cn.size, np.int32)
     st = Stats()
-    _check(lib().dger_sgns_train(C.byref(opts), _ptr(c), c.size, _ptr(cn), cn.size, _ptr(emb),
-                                 _ptr(rank_id), C.byref(st)))
+    if stream is None:
+        _check(lib().dger_sgns_train(C.byref(opts), _ptr(c), c.size, _ptr(cn), cn.size, _ptr(emb),
+                                     _ptr(rank_id), C.byref(st)))
+    else:
+        _check(lib().dger_sgns_train_stream(C.byref(opts), _ptr(c), c.size, _ptr(cn), cn.size, _ptr(emb),
+                                            _ptr(rank_id), C.byref(st), stream))
     return emb, rank_id, st
 
 

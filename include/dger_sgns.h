@@ -85,8 +85,12 @@ typedef struct dger_sgns_opts {
   int32_t device;          /* -device      HIP device ordinal (0) */
   int32_t graph;           /* -graph       1: the trainer launch (counter reset + kernel) is captured
                               once in a HIP graph and replayed by later passes (1) */
-  int32_t autotune;        /* -autotune    1: with -workers 0, the first pass times candidate
-                              worker counts on a corpus prefix and keeps the fastest (0) */
+  int32_t autotune;        /* -autotune    with -workers 0, the first pass times candidates on a
+                              corpus prefix and keeps the fastest: 1 worker counts and
+                              workgroup shapes; 2 (window-slot trainer) also -share 2/4/8 and
+                              the minibatch (one per step, or -batch-size 2*window+1 honoured),
+                              choices that change the training and that the caller gates on
+                              embedding quality (0) */
   int32_t rounding;        /* -rounding    bf16 row stores and Woh updates of the window-slot
                               trainer: sr (0) stochastic rounding, so that updates below half
                               a bf16 step survive in expectation (multi-epoch or small-alpha
@@ -94,6 +98,18 @@ typedef struct dger_sgns_opts {
   int32_t masters;         /* -masters     1: the window-slot trainer keeps FP32 master rows
                               for the top 1 % ranks by count in Wih and Woh, read and
                               updated in FP32 (0) */
+  int32_t batch_parity;    /* -batch-parity 1: -batch-size is replaced by 2*window+1 and a step's
+                              contexts form one minibatch, as in the reference (dsgl.hpp:832,
+                              :1163-1167); 0: -batch-size is honoured as the minibatch size: a
+                              step's contexts (first window, then second, in order) are trained
+                              in minibatches of at most batch_size contexts, each updating the
+                              output and window rows before the next is formed (window-slot
+                              trainer only) (1) */
+  int32_t share;           /* -share       target positions sharing one negative group: 2, the
+                              reference's two sentences per step (dsgl.hpp:1136-1158), or 4, 8,
+                              16 (window-slot trainer), where share/2 consecutive steps draw from
+                              one group and a refresh draws ceil(max(L1, L2) / (share/2)) groups
+                              (2) */
 } dger_sgns_opts;
 
 typedef struct dger_sgns_stats {
@@ -110,6 +126,8 @@ typedef struct dger_sgns_stats {
   uint64_t contexts;       /* sum over steps of the contexts of both windows */
   uint64_t out_ctx;        /* sum over steps of (output rows x contexts); each of the three
                               contractions is D * out_ctx multiply-adds */
+  int32_t share;           /* negative sharing the run used */
+  int32_t minibatch;       /* contexts per minibatch the run used; 0 = a step's contexts at once */
 } dger_sgns_stats;
 
 /* Fills *out with the reference defaults (dsgl.hpp:67-76) and GPU defaults. */
@@ -128,6 +146,15 @@ int dger_sgns_parse_args(int argc, char **argv, dger_sgns_opts *out);
 int dger_sgns_train(const dger_sgns_opts *opts, const int32_t *corpus, uint64_t n_tokens,
                     const int32_t *vertex_cn, uint32_t v_num, float *emb_out,
                     int32_t *rank_id_out, dger_sgns_stats *stats);
+/* dger_sgns_train with its device work -- uploads, trainer launches, model
+ * copies -- ordered on the caller's HIP stream (a hipStream_t passed as a
+ * pointer; NULL = the library's own stream), so that it follows work the
+ * caller queued there without a host wait.  Still blocking: the stream is
+ * synchronised before the call returns.  The stream must belong to
+ * opts->device (else DGER_E_ARG). */
+int dger_sgns_train_stream(const dger_sgns_opts *opts, const int32_t *corpus, uint64_t n_tokens,
+                           const int32_t *vertex_cn, uint32_t v_num, float *emb_out,
+                           int32_t *rank_id_out, dger_sgns_stats *stats, void *stream);
 
 /* The dsgl() drop-in (dsgl.hpp:1464-1584): parse argv, train, and when
  * -eoutput is given write the embedding file in my_saveModel's format
@@ -147,6 +174,12 @@ typedef struct dger_ctx dger_ctx;
  * model (dsgl.hpp:433-457) on the device. */
 int dger_ctx_create(const dger_sgns_opts *opts, const int32_t *vertex_cn, uint32_t v_num,
                     dger_ctx **out);
+/* Orders the context's later device work on the caller's HIP stream (a
+ * hipStream_t passed as a pointer; NULL = back to the context's own stream).
+ * Work already queued on the previous stream is waited for first.  The
+ * stream must belong to the context's device (else DGER_E_ARG) and outlive
+ * its use by the context. */
+int dger_ctx_set_stream(dger_ctx *ctx, void *stream);
 /* Copies the corpus to device memory (validated on the host first). */
 int dger_ctx_set_corpus(dger_ctx *ctx, const int32_t *corpus, uint64_t n_tokens);
 
@@ -185,8 +218,16 @@ int dger_ctx_set_model(dger_ctx *ctx, const float *wih, const float *woh);
 int dger_ctx_get_vocab(dger_ctx *ctx, int32_t *rank_id, uint64_t *train_words);
 /* Copies the noise table (table_size int32) to the host. */
 int dger_ctx_get_table(dger_ctx *ctx, int32_t *table);
-/* Writes Wih in my_saveModel's format (dsgl.hpp:1443-1461). */
+/* Writes Wih in my_saveModel's format (dsgl.hpp:1443-1461), through
+ * dger_emb_write. */
 int dger_ctx_save(dger_ctx *ctx, const char *path, int32_t binary);
+/* Writes v embedding rows (rows: v x d float32; ids[r]: the vertex id
+ * printed before row r) in my_saveModel's format (dsgl.hpp:1443-1461): a
+ * "v d" header line, then per row "id " followed by d values as "%f " text
+ * (the bytes printf writes, formatted without printf by several threads) or,
+ * with binary = 1, as raw float32, and a newline.  Host only (no device). */
+int dger_emb_write(const char *path, uint32_t v, int32_t d, const int32_t *ids, const float *rows,
+                   int32_t binary);
 void dger_ctx_destroy(dger_ctx *ctx);
 
 /* Width in int32 of one index-stream record:

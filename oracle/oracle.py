@@ -52,6 +52,7 @@ class OrcOpts(C.Structure):
         ("iter", C.c_int32), ("threads", C.c_int32), ("workers", C.c_int32),
         ("sample", C.c_float), ("alpha", C.c_float), ("table_size", C.c_int64),
         ("init_model", C.c_int32), ("parallel", C.c_int32), ("semantics", C.c_int32),
+        ("share_shift", C.c_int32), ("minibatch", C.c_int32),
     ]
 
 
@@ -175,11 +176,15 @@ SEMANTICS = {"reference": 0, "slots": 1}
 
 
 def make_opts(dim=128, window=10, negative=5, iter=1, threads=2, workers=1, sample=1e-3,
-              alpha=0.01, table_size=TABLE_SIZE, init_model=1, parallel=0, semantics="reference") -> OrcOpts:
+              alpha=0.01, table_size=TABLE_SIZE, init_model=1, parallel=0, semantics="reference",
+              share=2, minibatch=0) -> OrcOpts:
     """semantics: "reference" (dsgl.hpp's buffers) or "slots" (the update
-    rules of the library's BF16-MFMA window-slot trainer)."""
+    rules of the library's BF16-MFMA window-slot trainer).  share (2, 4, 8,
+    16: target positions per negative group) and minibatch (contexts per
+    minibatch, 0 = a step's at once) apply to "slots" only."""
+    shift = {2: 0, 4: 1, 8: 2, 16: 3}[share]
     return OrcOpts(dim, window, negative, iter, threads, workers, sample, alpha, table_size,
-                   init_model, parallel, SEMANTICS[semantics])
+                   init_model, parallel, SEMANTICS[semantics], shift, minibatch)
 
 
 def train(corpus: np.ndarray, cn_by_id: np.ndarray, opts: OrcOpts, wih=None, woh=None,

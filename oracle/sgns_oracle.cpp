@@ -247,6 +247,9 @@ typedef struct orc_opts {
   int32_t parallel;     // 0: workers run one after another; 1: OpenMP Hogwild
   int32_t semantics;    // 0: the reference's buffers; 1: window slots (the BF16-MFMA
                         //    window-slot trainer's update rules, see run_worker_slots)
+  int32_t share_shift;  // window slots only: 2^share_shift consecutive steps draw from
+                        //    one negative group (0 = the reference's one group per step)
+  int32_t minibatch;    // window slots only: contexts per minibatch (0 = a step at once)
 } orc_opts;
 
 typedef struct orc_stats {
@@ -523,6 +526,15 @@ WorkerResult run_worker(Shared& sh, int w, int32_t* rec, int64_t rec_cap) {
 //   * target and negative rows are read from Woh at the step and updated
 //     there directly (no negative buffer);
 //   * repeated contexts share their row: one coefficient times their count.
+// Two batching options change the training itself (the library's -share and
+// -batch-parity 0; with share_shift = minibatch = 0 the stream is the
+// reference's):
+//   * share_shift h: a refresh draws ceil(max(L1, L2) / 2^h) groups of K and
+//     step n uses group (n >> h) mod that count;
+//   * minibatch m > 0: the step's context list (first window, then second,
+//     in order) is split into runs of at most m; each run reads the output
+//     rows from Woh again and updates them and the working rows before the
+//     next run is formed.
 WorkerResult run_worker_slots(Shared& sh, int w, int32_t* rec, int64_t rec_cap) {
   const orc_opts& o = *sh.o;
   const Tables& t = *sh.t;
@@ -620,7 +632,7 @@ WorkerResult run_worker_slots(Shared& sh, int w, int32_t* rec, int64_t rec_cap) 
     }
     if (refresh) {
       neg_words.clear();
-      nbn = std::max(s[0].len, s[1].len);
+      nbn = (std::max(s[0].len, s[1].len) + (1 << o.share_shift) - 1) >> o.share_shift;
       for (int i = 0; i < nbn; i++)
         for (int k = 0; k < K; k++) {
           rng = lcg_next(rng);
@@ -634,10 +646,9 @@ WorkerResult run_worker_slots(Shared& sh, int w, int32_t* rec, int64_t rec_cap) 
     const int t2 = s[1].tok[s[1].pos];
     rng = lcg_next(rng);
     const int b = (int)(rng % (u64)W);
-    // contexts of both windows, as distinct rows with counts
-    std::vector<int> ctx_rank, ctx_q, ctx_e, ctx_mult;
+    // contexts of both windows in order: sentence and position
     int nin = 0, fin = 0;
-    std::vector<int> inputs;
+    std::vector<int> inputs, in_q, in_c;
     for (int q = 0; q < 2; q++) {
       const Win& x = s[q];
       for (int i = b; i < 2 * W + 1 - b; i++) {
@@ -646,17 +657,14 @@ WorkerResult run_worker_slots(Shared& sh, int w, int32_t* rec, int64_t rec_cap) 
         if (c < 0) continue;
         if (c >= x.len) break;
         inputs.push_back(x.tok[c]);
+        in_q.push_back(q);
+        in_c.push_back(c);
         nin++;
-        const int e = x.find(x.tok[c]);
-        size_t u = 0;
-        while (u < ctx_e.size() && !(ctx_q[u] == q && ctx_e[u] == e)) u++;
-        if (u == ctx_e.size()) { ctx_q.push_back(q); ctx_e.push_back(e); ctx_mult.push_back(0); }
-        ctx_mult[u]++;
       }
       if (q == 0) fin = nin;
     }
     neg_count++;
-    const int grp = neg_count % nbn;
+    const int grp = (int)(((unsigned)neg_count >> o.share_shift) % (unsigned)nbn);
     int os = 1;
     oidx[0] = t1;
     ometa[0] = 1;
@@ -681,7 +689,6 @@ WorkerResult run_worker_slots(Shared& sh, int w, int32_t* rec, int64_t rec_cap) 
         std::memcpy(in_row[q].data(), sh.wih + (size_t)x.tok[y] * D, sizeof(float) * D);
       }
     }
-    for (int i = 0; i < os; i++) std::memcpy(&O[(size_t)i * D], sh.woh + (size_t)oidx[i] * D, sizeof(float) * D);
     if (rec && (int64_t)res.steps < rec_cap) {
       int32_t* r = rec + (size_t)res.steps * stride;
       for (int z = 0; z < stride; z++) r[z] = -1;
@@ -691,6 +698,20 @@ WorkerResult run_worker_slots(Shared& sh, int w, int32_t* rec, int64_t rec_cap) 
       for (int j = 0; j < nin; j++) r[8 + j] = inputs[j];
       for (int i = 0; i < os; i++) { r[8 + 4 * W + i] = oidx[i]; r[8 + 4 * W + K + 2 + i] = ometa[i]; }
     }
+    const int mbs = o.minibatch > 0 ? o.minibatch : std::max(nin, 1);
+    for (int jb = 0; jb < std::max(nin, 1); jb += mbs) {
+    const int je = std::min(nin, jb + mbs);
+    // this minibatch's contexts as distinct rows with counts
+    std::vector<int> ctx_q, ctx_e, ctx_mult;
+    for (int j = jb; j < je; j++) {
+      const int q = in_q[j];
+      const int e = s[q].find(s[q].tok[in_c[j]]);
+      size_t u = 0;
+      while (u < ctx_e.size() && !(ctx_q[u] == q && ctx_e[u] == e)) u++;
+      if (u == ctx_e.size()) { ctx_q.push_back(q); ctx_e.push_back(e); ctx_mult.push_back(0); }
+      ctx_mult[u]++;
+    }
+    for (int i = 0; i < os; i++) std::memcpy(&O[(size_t)i * D], sh.woh + (size_t)oidx[i] * D, sizeof(float) * D);
     // coefficients over the distinct context rows (count times the table
     // gradient), GEMM2 on the rows before the update, then GEMM3
     const size_t nu = ctx_e.size();
@@ -721,6 +742,7 @@ WorkerResult run_worker_slots(Shared& sh, int w, int32_t* rec, int64_t rec_cap) 
     for (int i = 0; i < os; i++) {
       float* dst = sh.woh + (size_t)oidx[i] * D;
       for (int d = 0; d < D; d++) dst[d] += md[(size_t)i * D + d];
+    }
     }
     res.steps++;
     // transition
@@ -784,6 +806,8 @@ int orc_train(const orc_opts* o, const int32_t* corpus, uint64_t n_tokens, const
               int32_t V, float* wih, float* woh, int32_t* rec, int64_t rec_cap,
               uint64_t* worker_steps, orc_stats* stats) {
   if (o->workers < 1 || o->iter < 1) return -1;
+  if (o->share_shift < 0 || o->share_shift > 3 || o->minibatch < 0) return -1;
+  if (!o->semantics && (o->share_shift || o->minibatch)) return -1;  // the reference batches one way
   for (uint64_t i = 0; i < n_tokens; i++)
     if (corpus[i] < -1 || corpus[i] >= V) return -2;
   Tables t;

@@ -63,6 +63,16 @@ def test_parse_args_gpu_options():
         assert G.parse_args(["prog", "-precision", str(code)]).precision == code
 
 
+def test_parse_args_batching_options():
+    d = G.default_opts()
+    # parity defaults: -batch-size replaced by 2*window+1 (dsgl.hpp:832), two
+    # target positions per negative group (dsgl.hpp:1136-1158)
+    assert d.batch_parity == 1 and d.share == 2 and d.rounding == 1 and d.masters == 0
+    o = G.parse_args(["prog", "-batch-parity", "0", "-batch-size", "7", "-share", "4", "-autotune", "2",
+                      "-rounding", "sr", "-masters", "1"])
+    assert (o.batch_parity, o.batch_size, o.share, o.autotune, o.rounding, o.masters) == (0, 7, 4, 2, 0, 1)
+
+
 def test_parse_args_missing_value():
     with pytest.raises(G.DgerError) as e:
         G.parse_args(["prog", "-window"])
@@ -81,7 +91,10 @@ def test_bad_options_rejected_before_device():
                      (dict(size=66, precision="fp8"), G.DGER_E_UNSUPPORTED),
                      (dict(size=64, window=17, precision="fp8"), G.DGER_E_UNSUPPORTED),
                      (dict(size=64, negative=7, precision="fp8"), G.DGER_E_UNSUPPORTED),
-                     (dict(precision=9), G.DGER_E_ARG)]:
+                     (dict(precision=9), G.DGER_E_ARG),
+                     (dict(share=3), G.DGER_E_ARG), (dict(share=32), G.DGER_E_ARG),
+                     (dict(batch_parity=2), G.DGER_E_ARG), (dict(batch_parity=0, batch_size=0), G.DGER_E_ARG),
+                     (dict(autotune=3), G.DGER_E_ARG)]:
         with pytest.raises(G.DgerError) as e:
             G.Trainer(G.default_opts(**kw), cn)
         assert e.value.code == code, kw

@@ -200,3 +200,23 @@ def test_step_labels_by_hand():
         [(0 - s) * 0.5] * 2 + [(1 - s) * 0.5] * 3,
     ], np.float32)
     assert np.array_equal(corr, expect)
+
+
+def test_trace_negative_sharing():
+    # share 4 (two steps per negative group): the refresh draws
+    # ceil(max(3, 2) / 2) = 2 groups of K=1, then one window draw per step;
+    # step n (neg_count n) uses group (n >> 1) mod 2 (window-slot rules, whose
+    # index stream is the reference's at share 2)
+    corpus = [1, 2, 3, -1, 4, 5, -1, 6, 7, -1, 1, 3, -1]
+    for share, ngrp in ((2, 3), (4, 2), (8, 1)):
+        st, (recs,) = run(corpus, CN8, dim=8, window=1, negative=1, sample=0.0, table_size=10,
+                          semantics="slots", share=share)
+        s = lcg_seq(0, ngrp + 3)
+        neg = [x % 7 + 1 for x in s[:ngrp]]
+        shift = {2: 0, 4: 1, 8: 2}[share]
+        assert len(recs) == 3 and st.steps == 3
+        for n, (r, t1, t2) in enumerate(zip(recs, (1, 2, 3), (4, 5, 6)), start=1):
+            g = (n >> shift) % ngrp
+            assert r["grp"] == g and r["b"] == 0 and (r["t1"], r["t2"]) == (t1, t2)
+            v = neg[g]
+            assert r["outs"] == [t1] + ([v] if v != t1 else []) + [t2]

@@ -47,3 +47,42 @@ def test_mutated_oracle_semantics_fail_the_traces():
     assert not np.array_equal(case["wih"][2], W[2] - a * d[3])
     case = U.case_slice_end_drops_buffers()
     assert not np.array_equal(case["woh"][5], d[5] - a * (W[2] + W[4]))
+
+
+def test_slots_minibatch_trace():
+    """Window-slot update rules (the BF16-MFMA trainer's), K = 1, window 1,
+    sen1 = [1, 2], sen2 = [3, 4]; negatives 5 and 5 (groups 1 then 0).  Every
+    coefficient is 0 (label 1, saturated) or -a.
+
+    One minibatch per step: step 1 (t1 = 1, negative 5, t2 = 3; contexts 2
+    and 4) reads every output row before any update; step 2 (t1 = 2, 5,
+    t2 = 4; contexts 1 and 3) reads Woh after step 1.  Minibatches of one
+    context: the second context of a step reads the output rows the first one
+    updated, so context 4 sees Woh[5] = d5 - a W2 and context 3 sees
+    Woh[5] and Woh[2], Woh[4] after context 1's update.  Woh ends the same
+    (saturated coefficients do not depend on the rows)."""
+    W, d = U.initial_model()
+    a = U.ALPHA
+    corpus = np.array([1, 2, -1, 3, 4, -1] + U.PAD, np.int32)
+    woh = d.copy()
+    woh[1] = d[1] - a * W[4]
+    woh[3] = d[3] - a * W[2]
+    woh[5] = d[5] - a * (W[2] + W[4]) - a * (W[1] + W[3])
+    woh[2] = d[2] - a * W[3]
+    woh[4] = d[4] - a * W[1]
+    whole = W.copy()
+    whole[2] = W[2] - a * (d[5] + d[3])
+    whole[4] = W[4] - a * (d[1] + d[5])
+    whole[1] = W[1] - a * ((d[5] - a * (W[2] + W[4])) + d[4])
+    whole[3] = W[3] - a * (d[2] + (d[5] - a * (W[2] + W[4])))
+    split = whole.copy()
+    split[4] = W[4] - a * (d[1] + (d[5] - a * W[2]))
+    split[3] = W[3] - a * (d[2] + (d[5] - a * (W[2] + W[4]) - a * W[1]))
+    for mb, want in ((0, whole), (4, whole), (1, split)):
+        opts = O.make_opts(dim=U.D, window=1, negative=1, sample=0.0, alpha=float(a), table_size=10,
+                           init_model=0, semantics="slots", minibatch=mb)
+        wih, woh_got, st, _, _ = O.train(corpus, U.CN, opts, wih=W, woh=d)
+        assert st.steps == 2 and st.sentences == 2
+        assert np.array_equal(wih, want), (mb, wih, want)
+        assert np.array_equal(woh_got, woh), (mb, woh_got, woh)
+    assert not np.array_equal(split, whole)
