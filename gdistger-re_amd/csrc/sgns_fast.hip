@@ -10,11 +10,17 @@
 //   * Window slots.  The FP32 working row of every rank inside a sentence's
 //     window [p-W, p+W] sits in registers ("slots", 24 per sentence).  A row
 //     is read from Wih when its rank enters the window and written back to
-//     Wih (bf16, stochastic rounding) when its last occurrence leaves it or
-//     the sentence ends.  The reference keeps the whole sentence in an FP32
+//     Wih (bf16: round to nearest, or stochastic rounding with -rounding sr)
+//     when its last occurrence leaves it or the sentence ends.  The reference keeps the whole sentence in an FP32
 //     buffer and writes it back at the sentence end (dsgl.hpp:959-962,
 //     :1377-1395); here a rank that leaves the window and comes back later
 //     in the sentence is read again from Wih.
+//   * Slot copy.  A bf16 copy of every slot row sits in LDS (GEMM1 reads the
+//     slots across lanes from it).  In the instance without masters and
+//     stochastic rounding it is refreshed after each step's update and the
+//     rows entering the windows are written into it as they arrive, so the
+//     write-backs to Wih (round to nearest) read their rows from it, four
+//     rows per store instruction, instead of from the slots' registers.
 //   * Direct output updates.  Target and negative rows take their updates
 //     straight in Woh (bf16 atomic adds).  The reference adds target updates
 //     to Woh directly too (dsgl.hpp:1352-1356) but buffers negatives per
@@ -48,6 +54,7 @@
 // rows are read right after this step's updates (so they include them).
 
 #include <mutex>
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include "dger_sgns.h"
@@ -180,20 +187,31 @@ __device__ inline void occurrences(const int* v, int* first, int* prev, int* nex
   }
 }
 
-// The same with 16-bit outputs (positions are below kMaxSentence).
-__device__ inline void occurrences16(const int* v, short* first, short* prev, short* next, int n, int lane) {
+// The same with 16-bit outputs (positions are below kMaxSentence), the
+// first position only for p < nfirst.  The ranks are read into registers 64
+// at a time and broadcast with readlane (no LDS round trip per position).
+__device__ inline void occurrences16(const int* v, short* first, short* prev, short* next, int n, int nfirst, int lane) {
   for (int c0 = 0; c0 < n; c0 += 64) {
     const int p = c0 + lane;
-    const int x = (p < n) ? v[p] : 0;
-    int f = p, pv = -1, nx = kNone;
-#pragma unroll 8
-    for (int q = 0; q < n; q++) {
-      const bool eq = v[q] == x;
-      f = (eq && q < f) ? q : f;
-      pv = (eq && q < p) ? q : pv;
-      nx = (eq && q > p && nx == kNone) ? q : nx;
+    const int x = (p < n) ? v[p] : -1;
+    int pv = -1, nx = kNone;
+    for (int d0 = 0; d0 < n; d0 += 64) {
+      const int y = (d0 + lane < n) ? v[d0 + lane] : -2;
+      const int qn = min(64, n - d0);
+      for (int k = 0; k < qn; k++) {
+        const int yk = __builtin_amdgcn_readlane(y, k);
+        const int q = d0 + k;
+        const bool eq = yk == x;
+        pv = (eq && q < p) ? q : pv;              // ascending q: the last one below p
+        nx = (eq && q > p) ? min(nx, q) : nx;     // the smallest one above p
+      }
     }
-    if (p < n) { first[p] = (short)f; prev[p] = (short)pv; next[p] = (short)(nx == kNone ? kNone16 : nx); }
+    if (p < n) { prev[p] = (short)pv; next[p] = (short)(nx == kNone ? kNone16 : nx); }
+    if (c0 == 0) {
+      int f = p;
+      for (int k = nfirst - 1; k >= 0; k--) f = (__builtin_amdgcn_readlane(x, k) == x && k < f) ? k : f;
+      if (p < nfirst) first[p] = (short)f;
+    }
   }
 }
 
@@ -234,13 +252,61 @@ __device__ __forceinline__ unsigned mod64_recip(u64 x, unsigned d, double inv) {
 }
 
 // ---- the slots' registers -------------------------------------------------------
-struct Slots {
+typedef float f32x32 __attribute__((ext_vector_type(32)));
+// Two register layouts of the slots, the same values: SlotsV keeps each tile
+// as one 32-float vector, so that a slot (wave-uniform index) is read and
+// written by indexed register moves without branches; SlotsQ keeps the
+// MFMA quads apart and reaches a slot through a switch.  The instance
+// without masters and stochastic rounding uses SlotsV; the full instance,
+// whose other live registers leave no room for the tile vectors' indexed
+// writes, uses SlotsQ.
+struct SlotsV {
+  f32x32 T[3];  // T[rt][4 t + r]: element 8m + t of slot 16 rt + 4 g + r
+};
+struct SlotsQ {
   f32x4 R[3][NT];  // R[rt][t][r]: element 8m + t of slot 16 rt + 4 g + r
 };
+__device__ __forceinline__ float sget(const SlotsV& S, int rt, int t, int r) { return S.T[rt][4 * t + r]; }
+__device__ __forceinline__ float sget(const SlotsQ& S, int rt, int t, int r) { return S.R[rt][t][r]; }
+__device__ __forceinline__ f32x4 sget4(const SlotsV& S, int rt, int t) {
+  return (f32x4){S.T[rt][4 * t], S.T[rt][4 * t + 1], S.T[rt][4 * t + 2], S.T[rt][4 * t + 3]};
+}
+__device__ __forceinline__ f32x4 sget4(const SlotsQ& S, int rt, int t) { return S.R[rt][t]; }
+__device__ __forceinline__ void sset4(SlotsV& S, int rt, int t, const f32x4& c) {
+  S.T[rt][4 * t] = c[0]; S.T[rt][4 * t + 1] = c[1]; S.T[rt][4 * t + 2] = c[2]; S.T[rt][4 * t + 3] = c[3];
+}
+__device__ __forceinline__ void sset4(SlotsQ& S, int rt, int t, const f32x4& c) { S.R[rt][t] = c; }
+__device__ __forceinline__ void szero(SlotsV& S) {
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++) S.T[rt] = (f32x32)(0.f);
+}
+__device__ __forceinline__ void szero(SlotsQ& S) {
+#pragma unroll
+  for (int rt = 0; rt < 3; rt++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) S.R[rt][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+}
+
+// Slot r (wave-uniform) of one tile vector takes e in the lanes where own
+// holds (indexed moves under the lane mask).
+__device__ __forceinline__ f32x32 tile_put(f32x32 v, int r, const float (&e)[8], bool own) {
+  if (own) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) v[4 * t + r] = e[t];
+  }
+  return v;
+}
 
 // Slot s (wave-uniform) takes the row whose elements 8m .. 8m+7 this lane
-// holds in v; only the owner group's lanes change their registers.
-__device__ __forceinline__ void slot_put_f(Slots& S, int s, const float (&e)[8], int g) {
+// holds in e; only the owner group's lanes change their registers.
+__device__ __forceinline__ void slot_put_f(SlotsV& S, int s, const float (&e)[8], int g) {
+  const bool own = g == ((s >> 2) & 3);
+  const int rt = s >> 4, r = s & 3;
+  S.T[0] = tile_put(S.T[0], r, e, own && rt == 0);
+  S.T[1] = tile_put(S.T[1], r, e, own && rt == 1);
+  S.T[2] = tile_put(S.T[2], r, e, own && rt == 2);
+}
+__device__ __forceinline__ void slot_put_f(SlotsQ& S, int s, const float (&e)[8], int g) {
   const bool own = g == ((s >> 2) & 3);
   switch (((s >> 4) << 2) | (s & 3)) {
 #define DGER_PUT(RT, RR)                                             \
@@ -256,14 +322,24 @@ __device__ __forceinline__ void slot_put_f(Slots& S, int s, const float (&e)[8],
 }
 
 // The same from a bf16 row (elements 8m .. 8m+7 in v).
-__device__ __forceinline__ void slot_put(Slots& S, int s, const u32x4& v, int g) {
+template <class SL>
+__device__ __forceinline__ void slot_put(SL& S, int s, const u32x4& v, int g) {
   float e[8];
   unpack8(v, e);
   slot_put_f(S, s, e, g);
 }
 
 // Elements 8m .. 8m+7 of slot s held by this lane (meaningful in the owner group).
-__device__ __forceinline__ void slot_get(const Slots& S, int s, float (&e)[8]) {
+__device__ __forceinline__ void slot_get(const SlotsV& S, int s, float (&e)[8]) {
+  const int rt = s >> 4, r = s & 3;
+  const f32x32 t0 = S.T[0], t1 = S.T[1], t2 = S.T[2];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const float a = t0[4 * t + r], b = t1[4 * t + r], c = t2[4 * t + r];
+    e[t] = rt == 0 ? a : (rt == 1 ? b : c);
+  }
+}
+__device__ __forceinline__ void slot_get(const SlotsQ& S, int s, float (&e)[8]) {
   switch (((s >> 4) << 2) | (s & 3)) {
 #define DGER_GET(RT, RR)                                                     \
   case RT * 4 + RR:                                                          \
@@ -319,14 +395,15 @@ struct StepLds {
 };
 
 // bf16 copy of every slot row into rc (GEMM1 reads the slots across lanes).
-__device__ __forceinline__ void ring_copy(const Slots& S, unsigned char* rc, int g, int m) {
+template <class SL>
+__device__ __forceinline__ void ring_copy(const SL& S, unsigned char* rc, int g, int m) {
 #pragma unroll
   for (int rt = 0; rt < 3; rt++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       u32x4 v;
 #pragma unroll
-      for (int k = 0; k < 4; k++) v[k] = pack_rne(S.R[rt][2 * k][r], S.R[rt][2 * k + 1][r]);
+      for (int k = 0; k < 4; k++) v[k] = pack_rne(sget(S, rt, 2 * k, r), sget(S, rt, 2 * k + 1, r));
       *reinterpret_cast<u32x4*>(rc + swz(16 * rt + 4 * g + r, m)) = v;
     }
 }
@@ -395,17 +472,18 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
 // update of output row 4g + r, element 8m + t, from the coefficients in cb
 // (a0, a1: GEMM2's A operand, read once per step) and the slots' values
 // before this step's update.
-__device__ __forceinline__ void gemm2_half(const Slots& S, const bf16x8& a0, const bf16x8& a1, int t0,
+template <class SL>
+__device__ __forceinline__ void gemm2_half(const SL& S, const bf16x8& a0, const bf16x8& a1, int t0,
                                            f32x4 (&md)[NT / 2]) {
 #pragma unroll
   for (int t = t0; t < t0 + NT / 2; t++) {
     u32x4 b0, b1;
-    b0[0] = pack_rne(S.R[0][t][0], S.R[0][t][1]);
-    b0[1] = pack_rne(S.R[0][t][2], S.R[0][t][3]);
-    b0[2] = pack_rne(S.R[1][t][0], S.R[1][t][1]);
-    b0[3] = pack_rne(S.R[1][t][2], S.R[1][t][3]);
-    b1[0] = pack_rne(S.R[2][t][0], S.R[2][t][1]);
-    b1[1] = pack_rne(S.R[2][t][2], S.R[2][t][3]);
+    b0[0] = pack_rne(sget(S, 0, t, 0), sget(S, 0, t, 1));
+    b0[1] = pack_rne(sget(S, 0, t, 2), sget(S, 0, t, 3));
+    b0[2] = pack_rne(sget(S, 1, t, 0), sget(S, 1, t, 1));
+    b0[3] = pack_rne(sget(S, 1, t, 2), sget(S, 1, t, 3));
+    b1[0] = pack_rne(sget(S, 2, t, 0), sget(S, 2, t, 1));
+    b1[1] = pack_rne(sget(S, 2, t, 2), sget(S, 2, t, 3));
     b1[2] = 0u;
     b1[3] = 0u;
     f32x4 c = (f32x4){0.f, 0.f, 0.f, 0.f};
@@ -417,7 +495,8 @@ __device__ __forceinline__ void gemm2_half(const Slots& S, const bf16x8& a0, con
 // GEMM3: every slot row += sum over outputs of C[slot][i] O[i] (zero for
 // slots that are no context).  The B operand comes from the output rows in
 // ost: rows 4g .. 4g+3, elements 8m .. 8m+7.
-__device__ __forceinline__ void gemm3(Slots& S, const float (&C)[3][4], const unsigned char* ost, int g, int m) {
+template <class SL>
+__device__ __forceinline__ void gemm3(SL& S, const float (&C)[3][4], const unsigned char* ost, int g, int m) {
   u32x4 o3[4];
 #pragma unroll
   for (int e = 0; e < 4; e++) o3[e] = *reinterpret_cast<const u32x4*>(ost + swz(4 * g + e, m));
@@ -434,7 +513,7 @@ __device__ __forceinline__ void gemm3(Slots& S, const float (&C)[3][4], const un
     const unsigned hi = __builtin_amdgcn_perm(o3[3][t >> 1], o3[2][t >> 1], sel);
     const short4v b = __builtin_bit_cast(short4v, (u64)lo | ((u64)hi << 32));
 #pragma unroll
-    for (int rt = 0; rt < 3; rt++) S.R[rt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[rt], b, S.R[rt][t], 0, 0, 0);
+    for (int rt = 0; rt < 3; rt++) sset4(S, rt, t, __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[rt], b, sget4(S, rt, t), 0, 0, 0));
   }
 }
 
@@ -610,11 +689,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   if (lane < kSlots) { s.mult[lane] = 0; s.srank[lane] = 0; }
   if (lane < kStatCount) s.stats[lane] = 0;
   if (lane < 2 * 2 * kOutRows) s.plans[lane] = 0;
-  Slots S;
-#pragma unroll
-  for (int rt = 0; rt < 3; rt++)
-#pragma unroll
-    for (int t = 0; t < NT; t++) S.R[rt][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  typename std::conditional<kFull, SlotsQ, SlotsV>::type S;
+  szero(S);
   wave_sync();
 
   // Corpus slice and seed of compute thread wk+1 (dsgl.hpp:791, :811-821).
@@ -726,10 +802,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         if (L <= 1) continue;
         wave_sync();
         short* const sq = s.sop + q * Lmax;
-        occurrences16(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, lane);
-        wave_sync();
         // Slots of the first window [0, W]: the k-th distinct rank takes slot 24q + k.
         const int nw = min(W, L - 1) + 1;
+        occurrences16(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, nw, lane);
+        wave_sync();
         const bool isnew = lane < nw && s.prv[q * Lmax + lane] < 0;
         const u64 newmask = __ballot(isnew);
         const int nnew = __popcll(newmask);
@@ -743,9 +819,12 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         // Read the new slots' rows into LDS (the bf16 slot copy is rewritten
         // at the next step, so it serves as the staging area), then move
         // them into their slots.
+        // (the plain instance stages them in ost, which holds nothing
+        // between steps, and also writes them into the slot copy)
+        unsigned char* const stg = kFull ? s.rc : s.ost;
         for (int k = 0; k < nnew; k++) {
           const int rk = __builtin_amdgcn_readfirstlane(s.srank[kSentSlots * q + k]);
-          wih.load_lds((unsigned)rk * (D * 2) + 4 * lane, s.rc + k * (D * 2));
+          wih.load_lds((unsigned)rk * (D * 2) + 4 * lane, stg + k * (D * 2));
         }
         wait_vm_all();
         for (int k = 0; k < nnew; k++) {
@@ -755,7 +834,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
             master_get(wm, rk, m, e);
             slot_put_f(S, kSentSlots * q + k, e, g);
           } else {
-            slot_put(S, kSentSlots * q + k, *reinterpret_cast<const u32x4*>(s.rc + k * (D * 2) + 16 * m), g);
+            const u32x4 v = *reinterpret_cast<const u32x4*>(stg + k * (D * 2) + 16 * m);
+            slot_put(S, kSentSlots * q + k, v, g);
+            // (every group holds the row's 16 chunks: group 0 writes them)
+            if (!kFull && g == 0) *reinterpret_cast<u32x4*>(s.rc + swz(kSentSlots * q + k, m)) = v;
           }
         }
         wave_sync();
@@ -874,7 +956,9 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     if (!kFull || j0 == 0) neg_count++;
     const int grp = (int)mod32((unsigned)neg_count >> hs, divN);
     // The next step's group, read now for the planning after GEMM2.
-    const int vnext = (lane < K) ? negw[(int)mod32((unsigned)(neg_count + 1) >> hs, divN) * K + lane] : -1;
+    // (every lane loads, from a clamped index, so that the load is not
+    // waited for here; lanes >= K are masked at the use)
+    const int vnext = negw[(int)mod32((unsigned)(neg_count + 1) >> hs, divN) * K + min(lane, max(K - 1, 0))];
     auto make_outputs = [&](int pl, int ta, int tb, int v) -> int {
       int* const pr = s.plans + pl * 2 * kOutRows;
       const bool live = lane < K;
@@ -939,11 +1023,14 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
           slot_put_f(S, pend[q], e, g);
         } else {
           slot_put(S, pend[q], in_row[q], g);
+          if (!kFull && g == 0) *reinterpret_cast<u32x4*>(s.rc + swz(pend[q], m)) = in_row[q];
         }
         pend[q] = -1;
       }
     if (planned && nmaster > 0) out_rows_from_stage(o3, nmaster, cplan, os, g, m, ostage);
-    ring_copy(S, s.rc, g, m);
+    // The full instance copies the slots here; the plain one keeps the copy
+    // made at the end of the last step (plus the rows put since).
+    if (kFull) ring_copy(S, s.rc, g, m);
     // the output rows, bf16, for GEMM1's A operand and GEMM3's B operand
 #pragma unroll
     for (int e = 0; e < 4; e++) *reinterpret_cast<u32x4*>(s.ost + swz(4 * g + e, m)) = o3[e];
@@ -1054,7 +1141,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     int osn = 0;
     if (cont) {
       const int np = plan ^ 1;
-      osn = make_outputs(np, s.tok[p0 + 1], s.tok[Lmax + p1 + 1], vnext);
+      osn = make_outputs(np, s.tok[p0 + 1], s.tok[Lmax + p1 + 1], (lane < K) ? vnext : -1);
       load_output_l3(woh, om, nmaster, s.plans + np * 2 * kOutRows, osn, g, m, lane, o3, ostage);
       staged = nmaster > 0;
     } else {
@@ -1064,6 +1151,14 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       for (int e = 0; e < 4; e++) o3[e] = (u32x4){0u, 0u, 0u, 0u};
     }
     gemm3(S, C, s.ost, g, m);
+    if (!kFull) {
+      // bf16 copy of the updated slots: the write-backs below and the next
+      // step's GEMM1 read it (GEMM2's coefficients and the staged updates in
+      // rc have been read by now; LDS accesses of a wave run in order)
+      wave_sync();
+      ring_copy(S, s.rc, g, m);
+      wave_sync();
+    }
     // entering rows with FP32 masters: into ost, which GEMM3 no longer needs
 #pragma unroll
     for (int q = 0; q < 2; q++)
@@ -1099,6 +1194,22 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       short* const sq = s.sop + q * Lmax;
       pos[q] = p + 1;
       if (p + 1 >= Lq) {
+        if (!kFull) {
+          // from the slot copy, four rows per store (group g takes the g-th)
+          for (unsigned live = ~freem[q] & 0xFFFFFFu; live;) {
+            int sl4 = -1;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              const int b = live ? __builtin_ctz(live) : -1;
+              sl4 = (g == k && b >= 0) ? kSentSlots * q + b : sl4;
+              live &= live - 1;
+            }
+            if (sl4 >= 0) {
+              const int rk = s.srank[sl4];
+              wih.store((unsigned)rk * (D * 2) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sl4, m)));
+            }
+          }
+        } else
         for (unsigned live = ~freem[q] & 0xFFFFFFu; live; live &= live - 1) {
           const int sl = kSentSlots * q + __builtin_ctz(live);
           float e[8];
@@ -1117,12 +1228,17 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       if (x >= 0 && __builtin_amdgcn_readfirstlane(s.nxt[q * Lmax + x]) > y) {
         // x's rank leaves the window: write its slot back and free it.
         const int sx = __builtin_amdgcn_readfirstlane(sq[x]);
+        if (!kFull) {
+          if (g == q) wih.store((unsigned)__builtin_amdgcn_readfirstlane(tq[x]) * (D * 2) + 16 * m,
+                                *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
+        } else {
         float e[8];
         slot_get(S, sx, e);
         if (g == ((sx >> 2) & 3))
           write_back(wih, wm, is_master(__builtin_amdgcn_readfirstlane(tq[x])), __builtin_amdgcn_readfirstlane(tq[x]), m, e,
                      round_nearest,
                      salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sx << 24));
+        }
         freem[q] |= 1u << (sx - kSentSlots * q);
       }
       if (in_slot_src[q] >= 0) {
@@ -1176,13 +1292,13 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
   load_exp_table(expt, exp_table, lane, 64);
   if (lane < kSlots) ml[lane] = mult[lane];
   if (lane < kOutRows) { plan[lane] = lane; plan[kOutRows + lane] = lane < os ? meta[lane] : 0; }
-  Slots S;
+  SlotsV S;
 #pragma unroll
   for (int rt = 0; rt < 3; rt++)
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) S.R[rt][t][r] = I[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t];
+      for (int r = 0; r < 4; r++) S.T[rt][4 * (t) + (r)] = I[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t];
   // Output rows as bf16 in LDS (rows past os repeat row 0).
 #pragma unroll
   for (int e = 0; e < 4; e++) {
@@ -1226,7 +1342,7 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) I_out[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t] = S.R[rt][t][r];
+      for (int r = 0; r < 4; r++) I_out[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t] = S.T[rt][4 * (t) + (r)];
 }
 
 }  // namespace
