@@ -43,7 +43,8 @@ extern "C" {
 #define DGER_PRECISION_BF16_MFMA 2 /* BF16 rows, contractions on MFMA (bf16 operands, FP32 accumulate).
                                    At size 128, window <= 11, negative <= 14 it runs the window-slot
                                    trainer: window rows in registers, read from Wih on entering the
-                                   window and written back (stochastic rounding) on leaving it,
+                                   window and written back (round to nearest, or stochastic
+                                   rounding with -rounding sr) on leaving it,
                                    target and negative rows updated in Woh directly, all three
                                    contractions on MFMA; the index stream is the reference's.  Other
                                    shapes keep the reference's buffers with GEMM1 on MFMA. */

@@ -24,6 +24,15 @@ Pins (what checks each part, see tests/test_oracle_*.py):
                           tokens, and per-step invariants over random corpora.
   step_f32 / step_f64     float64 numpy matrix products of the same staged rows
                           (relative 1e-5), label layout checked on a hand case.
+  train (model updates)   hand-worked traces with exact dyadic arithmetic
+                          (tests/update_cases.py): reference semantics -- target
+                          updates in Woh, negative-buffer clobber, second
+                          sentence's write-back winning, slice-end drops;
+                          window-slot semantics ("slots") -- write-back when a
+                          rank leaves the window and re-read from Wih when it
+                          re-enters, negative updates in Woh, slice-end drop of
+                          the rows still in a window, minibatch order, negative
+                          sharing; traces that tell the two semantics apart.
 Parity unpinned: the std::sort tie order for more than 16 equal counts (it is
 libstdc++ introsort's, which the reference shares but which cannot be checked
 without a reference build), and the reference's MKL summation order (dsgl.hpp

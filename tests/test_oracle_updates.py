@@ -86,3 +86,36 @@ def test_slots_minibatch_trace():
         assert np.array_equal(wih, want), (mb, wih, want)
         assert np.array_equal(woh_got, woh), (mb, woh_got, woh)
     assert not np.array_equal(split, whole)
+
+
+@pytest.mark.parametrize("name", sorted(U.SLOT_CASES))
+def test_oracle_slot_semantics(name):
+    """The oracle's window-slot rules (semantics "slots", the BF16-MFMA
+    window-slot trainer's) against the hand-worked traces of
+    tests/update_cases.py."""
+    case = U.SLOT_CASES[name]()
+    wih0, woh0 = case["model"]()
+    opts = O.make_opts(dim=U.D, window=1, negative=case["negative"], sample=0.0,
+                       alpha=float(U.ALPHA), table_size=10, init_model=0, semantics="slots")
+    wih, woh, st, _, _ = O.train(np.array(case["corpus"], np.int32), case["cn"], opts, wih=wih0, woh=woh0)
+    assert st.steps == case["steps"] and st.sentences == case["sentences"]
+    assert np.array_equal(wih, case["wih"]), (wih, case["wih"])
+    assert np.array_equal(woh, case["woh"]), (woh, case["woh"])
+
+
+def test_slot_traces_stay_saturated_and_tell_rules_apart():
+    for name, make in U.SLOT_CASES.items():
+        case = make()
+        for m in (case["wih"], case["woh"]):
+            rows = m[1:]
+            assert np.all((rows[:, 0] >= 3) & (rows[:, 0] <= 4)), name
+            assert np.all(np.abs(rows[:, 1]) < 2), name
+    # keeping sen1's own copy of rank 1 across its absence from the window
+    # (instead of reading Wih again) would end elsewhere
+    case = U.slots_case_reentry_reads_wih()
+    assert not np.array_equal(case["wih"][1], case["wrong_wih1"])
+    # the reference's buffers on the slice-end corpus drop the negative
+    # updates and rank 3's row; the window-slot rules keep them
+    ref, slots = U.case_slice_end_drops_buffers(), U.slots_case_slice_end()
+    assert not np.array_equal(ref["woh"][5], slots["woh"][5])
+    assert not np.array_equal(ref["wih"][3], slots["wih"][3])
