@@ -169,22 +169,43 @@ __device__ __forceinline__ void wait_vm_all() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// For each position p < n: first, previous and next position holding the
-// same value (kNone / -1 where there is none).
-__device__ inline void occurrences(const int* v, int* first, int* prev, int* next, int n, int lane) {
+// For each position p < n: the previous and the next position holding the
+// same rank when it lies at most `reach` positions away (-1 / kNone16
+// otherwise), and for p < nfirst the first position of its rank.  The
+// trainer only asks whether a rank recurs within one window span
+// (2 window + 1 positions), so nearer occurrences are all it needs: each lane
+// compares its rank with the `reach` positions before it (LDS reads that do
+// not wait for each other) and writes itself as the next occurrence of the
+// one it finds.  The first positions (p < nfirst <= reach) follow the
+// previous-occurrence chain.
+__device__ inline void occurrences_near(const int* v, short* first, short* prev, short* next, int n, int nfirst,
+                                        int reach, int lane) {
+  for (int c0 = 0; c0 < n; c0 += 64)
+    if (c0 + lane < n) next[c0 + lane] = kNone16;
+  wave_sync();
   for (int c0 = 0; c0 < n; c0 += 64) {
     const int p = c0 + lane;
-    const int x = (p < n) ? v[p] : 0;
-    int f = p, pv = -1, nx = kNone;
-#pragma unroll 8
-    for (int q = 0; q < n; q++) {
-      const bool eq = v[q] == x;
-      f = (eq && q < f) ? q : f;
-      pv = (eq && q < p) ? q : pv;
-      nx = (eq && q > p && nx == kNone) ? q : nx;
+    const int x = (p < n) ? v[p] : -1;
+    int pv = -1;
+#pragma unroll 4
+    for (int d = 1; d <= reach; d++) {
+      const int q = p - d;
+      const int y = v[min(max(q, 0), n - 1)];
+      pv = (pv < 0 && q >= 0 && y == x) ? q : pv;
     }
-    if (p < n) { first[p] = f; prev[p] = pv; next[p] = nx; }
+    if (p < n) {
+      prev[p] = (short)pv;
+      if (pv >= 0) next[pv] = (short)p;
+    }
   }
+  wave_sync();
+  int f = lane;
+  for (int it = 0; it < nfirst; it++) {
+    const int q = (lane < nfirst) ? prev[f] : -1;
+    if (__ballot(q >= 0) == 0) break;
+    f = (q >= 0) ? q : f;
+  }
+  if (lane < nfirst) first[lane] = (short)f;
 }
 
 // The same with 16-bit outputs (positions are below kMaxSentence), the
@@ -276,6 +297,7 @@ __device__ __forceinline__ void sset4(SlotsV& S, int rt, int t, const f32x4& c) 
   S.T[rt][4 * t] = c[0]; S.T[rt][4 * t + 1] = c[1]; S.T[rt][4 * t + 2] = c[2]; S.T[rt][4 * t + 3] = c[3];
 }
 __device__ __forceinline__ void sset4(SlotsQ& S, int rt, int t, const f32x4& c) { S.R[rt][t] = c; }
+__device__ __forceinline__ void sset(SlotsV& S, int rt, int t, int r, float x) { S.T[rt][4 * t + r] = x; }
 __device__ __forceinline__ void szero(SlotsV& S) {
 #pragma unroll
   for (int rt = 0; rt < 3; rt++) S.T[rt] = (f32x32)(0.f);
@@ -717,6 +739,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   const Div32 divW = make_div32((unsigned)W);
   Div32 divN = make_div32(1u);
   u64 steps = 0;
+  u64 n_ctx = 0, n_outctx = 0;  // contexts and output-context pairs trained
   const int stride = record_stride(W, K);
   int32_t* const rec = a.rec ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
   int plan = 0;
@@ -804,7 +827,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         short* const sq = s.sop + q * Lmax;
         // Slots of the first window [0, W]: the k-th distinct rank takes slot 24q + k.
         const int nw = min(W, L - 1) + 1;
-        occurrences16(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, nw, lane);
+        if constexpr (kFull) occurrences16(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, nw, lane);
+        else occurrences_near(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, nw, 2 * W + 1, lane);
         wave_sync();
         const bool isnew = lane < nw && s.prv[q * Lmax + lane] < 0;
         const u64 newmask = __ballot(isnew);
@@ -822,11 +846,33 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         // (the plain instance stages them in ost, which holds nothing
         // between steps, and also writes them into the slot copy)
         unsigned char* const stg = kFull ? s.rc : s.ost;
-        for (int k = 0; k < nnew; k++) {
-          const int rk = __builtin_amdgcn_readfirstlane(s.srank[kSentSlots * q + k]);
-          wih.load_lds((unsigned)rk * (D * 2) + 4 * lane, stg + k * (D * 2));
-        }
+        const int krk = (lane < nnew) ? s.srank[kSentSlots * q + lane] : 0;  // rank of new row `lane`
+        for (int k = 0; k < nnew; k++)
+          wih.load_lds((unsigned)__builtin_amdgcn_readlane(krk, k) * (D * 2) + 4 * lane, stg + k * (D * 2));
         wait_vm_all();
+        if constexpr (!kFull) {
+          // All new rows at once: each lane fills those of its slot
+          // registers that are new slots of this sentence (fixed registers,
+          // a select per element), and the slot copy takes the rows four
+          // per store.
+#pragma unroll
+          for (int rt = 0; rt < 3; rt++) {
+            if ((q == 0 && rt == 2) || (q == 1 && rt == 0)) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              const int k = 16 * rt + 4 * g + r - kSentSlots * q;
+              const bool mine = k >= 0 && k < nnew;
+              const u32x4 v = *reinterpret_cast<const u32x4*>(stg + (mine ? k : 0) * (D * 2) + 16 * m);
+              float e[8];
+              unpack8(v, e);
+#pragma unroll
+              for (int t = 0; t < NT; t++) sset(S, rt, t, r, mine ? e[t] : sget(S, rt, t, r));
+            }
+          }
+          for (int k = g; k < nnew; k += 4)
+            *reinterpret_cast<u32x4*>(s.rc + swz(kSentSlots * q + k, m)) =
+                *reinterpret_cast<const u32x4*>(stg + k * (D * 2) + 16 * m);
+        } else
         for (int k = 0; k < nnew; k++) {
           const int rk = __builtin_amdgcn_readfirstlane(s.srank[kSentSlots * q + k]);
           if (is_master(rk)) {
@@ -834,10 +880,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
             master_get(wm, rk, m, e);
             slot_put_f(S, kSentSlots * q + k, e, g);
           } else {
-            const u32x4 v = *reinterpret_cast<const u32x4*>(stg + k * (D * 2) + 16 * m);
-            slot_put(S, kSentSlots * q + k, v, g);
-            // (every group holds the row's 16 chunks: group 0 writes them)
-            if (!kFull && g == 0) *reinterpret_cast<u32x4*>(s.rc + swz(kSentSlots * q + k, m)) = v;
+            slot_put(S, kSentSlots * q + k, *reinterpret_cast<const u32x4*>(stg + k * (D * 2) + 16 * m), g);
           }
         }
         wave_sync();
@@ -902,8 +945,23 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 
     // ---- one step (dsgl.hpp:1089-1368) ----
     const int p0 = pos[0], p1 = pos[1];
-    const int t1 = __builtin_amdgcn_readfirstlane(s.tok[p0]);
-    const int t2 = __builtin_amdgcn_readfirstlane(s.tok[Lmax + p1]);
+    // The step's scalar inputs in two gathers (one wait), then read lane by
+    // lane: gr = ranks at p, p+1, p-W, p+W+1 (lane 2k+q: sentence q, k-th of
+    // these), gs = previous occurrence of p+W+1 (lane q), next occurrence of
+    // p-W (lane 2+q) and slot of p-W (lane 4+q).  Positions are clamped into
+    // the sentence; the code below reads only those that exist.
+    int gr, gs;
+    {
+      const int q = lane & 1, k = (lane >> 1) & 3;
+      const int pq = q ? p1 : p0, last_pos = max(len[q], 1) - 1;
+      const int at = k == 0 ? pq : (k == 1 ? pq + 1 : (k == 2 ? pq - W : pq + W + 1));
+      gr = s.tok[q * Lmax + min(max(at, 0), last_pos)];
+      const int xs = min(max(pq - W, 0), last_pos), ys = min(pq + W + 1, last_pos);
+      const short* const arr = k == 0 ? s.prv : (k == 1 ? s.nxt : s.sop);
+      gs = arr[q * Lmax + (k == 0 ? ys : xs)];
+    }
+    const int t1 = __builtin_amdgcn_readlane(gr, 0);
+    const int t2 = __builtin_amdgcn_readlane(gr, 1);
     int b;
     if (!kFull || j0 == 0) {
       rng = rng * kLcgMul + kLcgAdd;
@@ -931,7 +989,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       const int x = pq - W, y = pq + W + 1;
       in_slot_src[q] = -2;  // nothing enters
       if (pq + 1 < Lq && y < Lq) {
-        const int pv = __builtin_amdgcn_readfirstlane(s.prv[q * Lmax + y]);
+        const int pv = __builtin_amdgcn_readlane(gs, q);
         in_slot_src[q] = (pv >= 0 && pv >= x) ? pv : -1;
       }
     }
@@ -1083,15 +1141,23 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       __attribute__((address_space(1))) float* const ob = (__attribute__((address_space(1))) float*)(a.omaster);
       if (plain) {
         // no masters, nearest rounding: bf16 pairs, all rows read first
-        unsigned v[kOutRows];
+        // (at most 8 output rows unless negative > 6: half the reads and tests)
+        auto add_rows = [&](auto nrows) {
+          constexpr int NR = decltype(nrows)::value;
+          unsigned v[NR];
 #pragma unroll
-        for (int i = 0; i < kOutRows; i++) v[i] = *reinterpret_cast<const unsigned*>(mdl + i * MDS + 4 * lane);
+          for (int i = 0; i < NR; i++) v[i] = *reinterpret_cast<const unsigned*>(mdl + i * MDS + 4 * lane);
 #pragma unroll
-        for (int i = 0; i < kOutRows; i++)
-          if (i < os) {
-            const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (D / 2);
-            (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + lane, __builtin_bit_cast(short2v, v[i]));
-          }
+          for (int i = 0; i < NR; i++)
+            if (i < os) {
+              const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (D / 2);
+              (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + lane, __builtin_bit_cast(short2v, v[i]));
+            }
+        };
+        if (os <= 8)
+          add_rows(std::integral_constant<int, 8>());
+        else
+          add_rows(std::integral_constant<int, kOutRows>());
       } else
       for (int i = 0; i < os; i++) {
         const int rk = __builtin_amdgcn_readlane(orank, i);
@@ -1130,7 +1196,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int y = (q ? p1 : p0) + W + 1;
-      const int rk = (last && in_slot_src[q] == -1) ? __builtin_amdgcn_readfirstlane(s.tok[q * Lmax + y]) : -1;
+      const int rk = (last && in_slot_src[q] == -1) ? __builtin_amdgcn_readlane(gr, 6 + q) : -1;
       pend_hot[q] = rk >= 0 && is_master(rk);  // staged into ost after GEMM3 below
       in_rk[q] = rk;
       if (!pend_hot[q]) in_row[q] = wih.load((unsigned)(rk >= 0 ? rk : 0) * (D * 2) + 16 * m);
@@ -1141,7 +1207,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     int osn = 0;
     if (cont) {
       const int np = plan ^ 1;
-      osn = make_outputs(np, s.tok[p0 + 1], s.tok[Lmax + p1 + 1], (lane < K) ? vnext : -1);
+      osn = make_outputs(np, __builtin_amdgcn_readlane(gr, 2), __builtin_amdgcn_readlane(gr, 3), (lane < K) ? vnext : -1);
       load_output_l3(woh, om, nmaster, s.plans + np * 2 * kOutRows, osn, g, m, lane, o3, ostage);
       staged = nmaster > 0;
     } else {
@@ -1168,10 +1234,9 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         staged = true;
       }
 
-    if (lane == 0) {
-      stat_add(s.stats, kStatCtx, (u64)(je - jb));
-      stat_add(s.stats, kStatOutCtx, (u64)(os * (je - jb)));
-    }
+    // (wave-uniform totals in registers: no lane mask, no LDS)
+    n_ctx += (u64)(je - jb);
+    n_outctx += (u64)(os * (je - jb));
     if (kFull && !last) {
       // The next minibatch of this step: its output rows are read again
       // (after the updates above); the windows stay.
@@ -1225,17 +1290,18 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         continue;
       }
       const int x = p - W, y = p + W + 1;
-      if (x >= 0 && __builtin_amdgcn_readfirstlane(s.nxt[q * Lmax + x]) > y) {
+      if (x >= 0 && __builtin_amdgcn_readlane(gs, 2 + q) > y) {
         // x's rank leaves the window: write its slot back and free it.
-        const int sx = __builtin_amdgcn_readfirstlane(sq[x]);
+        const int sx = __builtin_amdgcn_readlane(gs, 4 + q);
+        const int rkx = __builtin_amdgcn_readlane(gr, 4 + q);
         if (!kFull) {
-          if (g == q) wih.store((unsigned)__builtin_amdgcn_readfirstlane(tq[x]) * (D * 2) + 16 * m,
+          if (g == q) wih.store((unsigned)rkx * (D * 2) + 16 * m,
                                 *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
         } else {
         float e[8];
         slot_get(S, sx, e);
         if (g == ((sx >> 2) & 3))
-          write_back(wih, wm, is_master(__builtin_amdgcn_readfirstlane(tq[x])), __builtin_amdgcn_readfirstlane(tq[x]), m, e,
+          write_back(wih, wm, is_master(rkx), rkx, m, e,
                      round_nearest,
                      salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sx << 24));
         }
@@ -1249,7 +1315,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         freem[q] &= ~(1u << k);
         const int sl = kSentSlots * q + k;
         pend[q] = sl;
-        if (lane == 0) { sq[y] = (short)sl; s.srank[sl] = tq[y]; }
+        const int rky = __builtin_amdgcn_readlane(gr, 6 + q);
+        if (lane == 0) { sq[y] = (short)sl; s.srank[sl] = rky; }
       }
     }
     if (done_sent >= 2) {
@@ -1268,8 +1335,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     atomicAdd(&a.counters[kSteps], steps);
     atomicAdd(&a.counters[kRawWords], s.stats[kStatRaw]);
     atomicAdd(&a.counters[kSentences], s.stats[kStatSentences]);
-    atomicAdd(&a.counters[kContexts], s.stats[kStatCtx]);
-    atomicAdd(&a.counters[kOutCtx], s.stats[kStatOutCtx]);
+    atomicAdd(&a.counters[kContexts], n_ctx);
+    atomicAdd(&a.counters[kOutCtx], n_outctx);
   }
 }
 
