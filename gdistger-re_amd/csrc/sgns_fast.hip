@@ -740,6 +740,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   Div32 divN = make_div32(1u);
   u64 steps = 0;
   u64 n_ctx = 0, n_outctx = 0;  // contexts and output-context pairs trained
+  // Plain instance: lane s < 48 holds in bit j (j = 0 .. 2W) whether window
+  // position p - W + j of slot s's sentence holds slot s's rank (p = the
+  // sentence's current position).
+  unsigned pm = 0;
   const int stride = record_stride(W, K);
   int32_t* const rec = a.rec ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
   int plan = 0;
@@ -837,6 +841,13 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         if (lane < nw) myslot = kSentSlots * q + __popcll(newmask & lanemask_lt(sq[lane]));
         wave_sync();
         if (lane < nw) sq[lane] = (short)myslot;
+        if constexpr (!kFull) {
+          pm = (lane >= kSentSlots * q && lane < kSentSlots * (q + 1)) ? 0u : pm;
+          for (int p = 0; p < nw; p++) {
+            const int sl = __builtin_amdgcn_readlane(myslot, p);
+            pm |= (lane == sl) ? (1u << (p + W)) : 0u;
+          }
+        }
         if (isnew) s.srank[myslot] = tq[lane];
         freem[q] = 0xFFFFFFu & ~((1u << nnew) - 1u);
         wave_sync();
@@ -994,17 +1005,31 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       }
     }
 
-    // Context multiplicity per slot.
-    if (lane < kSlots) s.mult[lane] = 0;
-    wave_sync();
+    // Context multiplicity per slot.  Plain instance: the slot's window
+    // positions within the step's window, minus the centre (positions
+    // outside the sentence are never set).
     int my_ctx_rank = -1;
-    if (lane < nin) {
-      const int q = (lane < n0) ? 0 : 1;
-      const int jj = q ? lane - n0 : lane;
-      const int lo = q ? lo1 : lo0, pp = q ? p1 : p0;
-      const int c = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
-      if (lane >= jb && lane < je) atomicAdd(&s.mult[s.sop[q * Lmax + c]], 1);
-      my_ctx_rank = s.tok[q * Lmax + c];
+    if constexpr (!kFull) {
+      const unsigned ctxm = ((((2u << (2 * hw)) - 1u) << (W - hw)) & ~(1u << W));
+      if (lane < kSlots) s.mult[lane] = __builtin_popcount(pm & ctxm);
+      if (rec && lane < nin) {
+        const int q = (lane < n0) ? 0 : 1;
+        const int jj = q ? lane - n0 : lane;
+        const int lo = q ? lo1 : lo0, pp = q ? p1 : p0;
+        const int c = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
+        my_ctx_rank = s.tok[q * Lmax + c];
+      }
+    } else {
+      if (lane < kSlots) s.mult[lane] = 0;
+      wave_sync();
+      if (lane < nin) {
+        const int q = (lane < n0) ? 0 : 1;
+        const int jj = q ? lane - n0 : lane;
+        const int lo = q ? lo1 : lo0, pp = q ? p1 : p0;
+        const int c = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
+        if (lane >= jb && lane < je) atomicAdd(&s.mult[s.sop[q * Lmax + c]], 1);
+        my_ctx_rank = s.tok[q * Lmax + c];
+      }
     }
 
     // Output list (dsgl.hpp:1136-1158): target, distinct negatives of the
@@ -1251,6 +1276,11 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 
     // Transition: a finished sentence writes its live slots back; a
     // continuing one slides its window by one position.
+    unsigned pm_old = 0;
+    if constexpr (!kFull) {
+      pm_old = pm;
+      pm >>= 1;  // both windows move on by one position
+    }
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int p = pos[q];
@@ -1285,11 +1315,40 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         }
         freem[q] = 0xFFFFFFu;
         len[q] = 0;
+        if constexpr (!kFull) pm = (lane >= kSentSlots * q && lane < kSentSlots * (q + 1)) ? 0u : pm;
         done_sent++;
         if (lane == 0) stat_add(s.stats, kStatSentences, 1);
         continue;
       }
       const int x = p - W, y = p + W + 1;
+      if constexpr (!kFull) {
+        // x's slot: the one holding the position that just left (bit 0);
+        // a shared entering position takes its previous occurrence's slot;
+        // x's rank leaves if its slot holds no position any more.
+        const bool inq = lane >= kSentSlots * q && lane < kSentSlots * (q + 1);
+        int sx = 0;
+        if (x >= 0) sx = (int)__builtin_ctzll(__ballot(inq && (pm_old & 1u)));
+        if (in_slot_src[q] >= 0) {
+          const int j = in_slot_src[q] - x - 1;  // bit of that occurrence after the slide
+          const int sy = j < 0 ? sx : (int)__builtin_ctzll(__ballot(inq && ((pm >> j) & 1u)));
+          pm |= (lane == sy) ? (1u << (2 * W)) : 0u;
+        }
+        if (x >= 0 && __builtin_amdgcn_readlane(pm, sx) == 0) {
+          const int rkx = __builtin_amdgcn_readlane(gr, 4 + q);
+          if (g == q) wih.store((unsigned)rkx * (D * 2) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
+          freem[q] |= 1u << (sx - kSentSlots * q);
+        }
+        if (in_slot_src[q] == -1) {
+          const int k = __ffs((int)freem[q]) - 1;
+          freem[q] &= ~(1u << k);
+          const int sl = kSentSlots * q + k;
+          pend[q] = sl;
+          pm |= (lane == sl) ? (1u << (2 * W)) : 0u;
+          const int rky = __builtin_amdgcn_readlane(gr, 6 + q);
+          if (lane == 0) s.srank[sl] = rky;
+        }
+        continue;
+      }
       if (x >= 0 && __builtin_amdgcn_readlane(gs, 2 + q) > y) {
         // x's rank leaves the window: write its slot back and free it.
         const int sx = __builtin_amdgcn_readlane(gs, 4 + q);
