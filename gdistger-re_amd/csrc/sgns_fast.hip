@@ -1,11 +1,16 @@
 // gfx950 window-slot trainer of the BF16-MFMA form (DistGER DSGL,
 // include/dsgl.hpp:786-1408).
 //
-// One 64-lane wavefront per Hogwild worker, as in sgns_kernels.hip: worker k
-// trains the corpus slice and LCG stream of the reference's compute thread
-// k+1, so the index stream (sentences, subsampling, windows, negative groups,
-// output lists, learning rate) is the reference's.  What differs is where the
-// rows live and how a step is computed:
+// One 64-lane wavefront per Hogwild worker and row length 128, as in
+// sgns_kernels.hip: worker k trains the corpus slice and LCG stream of the
+// reference's compute thread k+1, so the index stream (sentences,
+// subsampling, windows, negative groups, output lists, learning rate) is the
+// reference's.  Rows of length 256 take two wavefronts per worker (one
+// workgroup), each holding one 128-element half of every row: both run the
+// same index stream, each reads, updates and writes back its half, and GEMM1's
+// partial dot products of the two halves are added through LDS (two workgroup
+// barriers per step).  What differs from the reference is where the rows live
+// and how a step is computed:
 //
 //   * Window slots.  The FP32 working row of every rank inside a sentence's
 //     window [p-W, p+W] sits in registers ("slots", 24 per sentence).  A row
@@ -38,7 +43,7 @@
 //       GEMM3  I  += C O           v_mfma_f32_16x16x16_bf16, 3 x 8, the
 //                                  slots' registers as the accumulator.
 //
-// Register layout (D = 128, lane = 16 g + m):
+// Register layout (D = 128 elements per wavefront, lane = 16 g + m):
 //   slot s = 16 rt + 4 g' + r lives in the lanes of group g' = (s >> 2) & 3;
 //   lane (g', m) holds elements 8m .. 8m+7 of it in R[rt][t][r], t = 0..7.
 // This is the output layout of GEMM3 (16x16x16: D[4g+r][n] for rows = slots,
@@ -71,7 +76,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int D = kFastDim;           // row length (128)
+constexpr int D = kFastDim;           // row elements per wavefront (128)
 constexpr int NT = D / 16;            // element tiles of GEMM2 / GEMM3 (8)
 constexpr int KB1 = D / 32;           // K blocks of GEMM1 (4)
 constexpr int kSlots = 48;            // 3 MFMA tiles of 16 slots
@@ -169,20 +174,14 @@ __device__ __forceinline__ void wait_vm_all() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// For each position p < n: the previous and the next position holding the
-// same rank when it lies at most `reach` positions away (-1 / kNone16
-// otherwise), and for p < nfirst the first position of its rank.  The
-// trainer only asks whether a rank recurs within one window span
-// (2 window + 1 positions), so nearer occurrences are all it needs: each lane
-// compares its rank with the `reach` positions before it (LDS reads that do
-// not wait for each other) and writes itself as the next occurrence of the
-// one it finds.  The first positions (p < nfirst <= reach) follow the
-// previous-occurrence chain.
-__device__ inline void occurrences_near(const int* v, short* first, short* prev, short* next, int n, int nfirst,
-                                        int reach, int lane) {
-  for (int c0 = 0; c0 < n; c0 += 64)
-    if (c0 + lane < n) next[c0 + lane] = kNone16;
-  wave_sync();
+// For each position p < n: the previous position holding the same rank when
+// it lies at most `reach` positions away (-1 otherwise); returns, in lane
+// p < nfirst, the first position of p's rank.  The trainer only asks whether
+// a rank recurs within one window span (2 window + 1 positions), so nearer
+// occurrences are all it needs: each lane compares its rank with the `reach`
+// positions before it (LDS reads that do not wait for each other).  The
+// first positions (p < nfirst <= reach) follow the previous-occurrence chain.
+__device__ inline int occurrences_near(const int* v, short* prev, int n, int nfirst, int reach, int lane) {
   for (int c0 = 0; c0 < n; c0 += 64) {
     const int p = c0 + lane;
     const int x = (p < n) ? v[p] : -1;
@@ -193,10 +192,7 @@ __device__ inline void occurrences_near(const int* v, short* first, short* prev,
       const int y = v[min(max(q, 0), n - 1)];
       pv = (pv < 0 && q >= 0 && y == x) ? q : pv;
     }
-    if (p < n) {
-      prev[p] = (short)pv;
-      if (pv >= 0) next[pv] = (short)p;
-    }
+    if (p < n) prev[p] = (short)pv;
   }
   wave_sync();
   int f = lane;
@@ -205,7 +201,7 @@ __device__ inline void occurrences_near(const int* v, short* first, short* prev,
     if (__ballot(q >= 0) == 0) break;
     f = (q >= 0) ? q : f;
   }
-  if (lane < nfirst) first[lane] = (short)f;
+  return f;
 }
 
 // The same with 16-bit outputs (positions are below kMaxSentence), the
@@ -413,7 +409,6 @@ __device__ __forceinline__ u32x4 pack8_rne(const float (&e)[8]) {
 struct StepLds {
   unsigned char* rc;
   unsigned char* cb;
-  const int* mult;  // contexts of the step per slot
 };
 
 // bf16 copy of every slot row into rc (GEMM1 reads the slots across lanes).
@@ -435,9 +430,18 @@ __device__ __forceinline__ void ring_copy(const SL& S, unsigned char* rc, int g,
 // On return C[rt][r] = coefficient of slot 16 rt + m and
 // output 4 g + r (0 unless the slot is a context and the output exists), cb
 // holds the coefficients for GEMM2.  meta4[r] = multiplicity of output 4g+r
-// (0 past the step's outputs); os = number of outputs.
+// (0 past the step's outputs); os = number of outputs; cnt (lane s < 48) =
+// the step's contexts in slot s.
+// kH = 2 (rows of two halves, one per wavefront of the workgroup): the
+// products over this half are written to xo (3 KB at rc + kMdOff, free
+// until GEMM2's staged updates), the other half's are read from xp, and both
+// wavefronts add them in the same order (own + other is exact-commutative),
+// so they get the same F.  The second barrier keeps xo until the other
+// wavefront has read it.
+template <int kH = 1>
 __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepLds& L, const int (&meta4)[4], int os,
-                                           float alpha, const float* expt, int g, int m, float (&C)[3][4]) {
+                                           float alpha, const float* expt, int g, int m, int cnt, float (&C)[3][4],
+                                           unsigned char* xo = nullptr, const unsigned char* xp = nullptr) {
   f32x4 f[3];
 #pragma unroll
   for (int rt = 0; rt < 3; rt++) f[rt] = (f32x4){0.f, 0.f, 0.f, 0.f};
@@ -450,6 +454,15 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
       f[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, f[rt], 0, 0, 0);
     }
   }
+  if constexpr (kH > 1) {
+    const int lane = 16 * g + m;
+#pragma unroll
+    for (int rt = 0; rt < 3; rt++) *reinterpret_cast<f32x4*>(xo + rt * 1024 + 16 * lane) = f[rt];
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < 3; rt++) f[rt] = f[rt] + *reinterpret_cast<const f32x4*>(xp + rt * 1024 + 16 * lane);
+    __syncthreads();
+  }
   // f[rt][r] = F[slot 16 rt + m][output 4 g + r].  The sigmoid is one
   // table read: the bin of x, or the entries 1 (x > 6) and 0 (x < -6) after
   // the table, so that no branch is needed.  All reads are issued before the
@@ -459,7 +472,7 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
   for (int r = 0; r < 4; r++) me[r] = (float)meta4[r];
 #pragma unroll
   for (int rt = 0; rt < 3; rt++) {
-    mu[rt] = (float)L.mult[16 * rt + m];
+    mu[rt] = (float)__builtin_amdgcn_ds_bpermute(4 * (16 * rt + m), cnt);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const float x = f[rt][r];
@@ -539,8 +552,10 @@ __device__ __forceinline__ void gemm3(SL& S, const float (&C)[3][4], const unsig
   }
 }
 
-// Per-worker totals in LDS: contexts, output-context pairs, sentences, raw words.
-enum { kStatCtx = 0, kStatOutCtx, kStatSentences, kStatRaw, kStatCount };
+// Per-worker totals in LDS: contexts, output-context pairs, sentences, raw
+// words; kStatXfer passes the learning rate's word total between the halves
+// of a row.
+enum { kStatCtx = 0, kStatOutCtx, kStatSentences, kStatRaw, kStatXfer, kStatCount };
 // One lane adds v to a per-worker total (an LDS atomic without return).
 __device__ __forceinline__ void stat_add(u64* stats, int k, u64 v) {
   __hip_atomic_fetch_add(stats + k, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
@@ -553,12 +568,13 @@ struct FastLds {
                       // has read it) the coefficients cb
   unsigned char* ost; // kOutRows x RB: the step's output rows (bf16, swizzled)
   unsigned char* cb;  // kOutRows x CBS, inside rc
-  int* mult;          // kSlots
   int* srank;         // kSlots: rank held by each slot
   int* plans;         // two plans: ranks [0, 16), multiplicities [16, 32)
   int* tok;           // 2 x Lmax ranks
-  short* prv;         // 2 x Lmax
-  short* nxt;         // 2 x Lmax
+  short* prv;         // 2 x Lmax: previous occurrence of each position's rank
+  // full instance only (the other tracks its windows in registers):
+  int* mult;          // kSlots: the step's contexts per slot
+  short* nxt;         // 2 x Lmax: next occurrence
   short* sop;         // 2 x Lmax: slot of each window position
   u64* stats;         // kStatCount per-worker totals (kept here, not in scalar registers)
 };
@@ -570,35 +586,39 @@ static_assert(kMdOff + kOutRows * MDS <= kSlots * RB, "the staged updates live i
 static_assert(kStageOff >= kSentSlots / 2 * RB && kStageOff + kOutRows * 4 * D <= kSlots * RB,
               "the staged master rows live inside the slot copy, past the first-window rows");
 static_assert(2 * 4 * D <= kOutRows * RB, "two entering master rows fit in the output rows' space");
+static_assert(kMdOff + 3 * 1024 <= kSlots * RB, "GEMM1's half-row products live inside the slot copy");
 
-__host__ __device__ inline size_t fast_layout(int Lmax, int K, size_t* off) {
+// One wavefront's LDS bytes; `full`: the instance with masters, stochastic
+// rounding and minibatches, which also keeps the context counts, next
+// occurrences and window slots in LDS.
+__host__ __device__ inline size_t fast_layout(int Lmax, int K, bool full, size_t* off) {
   (void)K;
   size_t o = 0;
   off[0] = o; o = al16(o + (size_t)(kSlots + kOutRows) * RB);
   off[1] = o;
-  off[2] = o; o = al16(o + 2 * (size_t)kSlots * 4);
+  off[2] = o; o = al16(o + (full ? 2 : 1) * (size_t)kSlots * 4);
   off[3] = o; o = al16(o + 2 * 2 * (size_t)kOutRows * 4);
   off[4] = o; o = al16(o + 2 * (size_t)Lmax * 4);
-  off[5] = o; o = al16(o + 3 * 2 * (size_t)Lmax * 2);
+  off[5] = o; o = al16(o + (full ? 3 : 1) * 2 * (size_t)Lmax * 2);
   off[1] = o; o = al16(o + kStatCount * 8);
   return o;
 }
 
-__device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K) {
+__device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K, bool full) {
   size_t off[6];
-  fast_layout(Lmax, K, off);
+  fast_layout(Lmax, K, full, off);
   FastLds s;
   s.rc = base + off[0];
   s.ost = base + off[0] + (size_t)kSlots * RB;
   s.cb = s.rc;
-  s.mult = reinterpret_cast<int*>(base + off[2]);
-  s.srank = s.mult + kSlots;
+  s.srank = reinterpret_cast<int*>(base + off[2]);
+  s.mult = full ? s.srank + kSlots : nullptr;
   s.plans = reinterpret_cast<int*>(base + off[3]);
   s.tok = reinterpret_cast<int*>(base + off[4]);
   short* b5 = reinterpret_cast<short*>(base + off[5]);
   s.prv = b5;
-  s.nxt = b5 + 2 * Lmax;
-  s.sop = b5 + 4 * Lmax;
+  s.nxt = full ? b5 + 2 * Lmax : nullptr;
+  s.sop = full ? b5 + 4 * Lmax : nullptr;
   s.stats = reinterpret_cast<u64*>(base + off[1]);
   return s;
 }
@@ -611,24 +631,33 @@ __device__ inline void load_exp_table(float* expt, const float* src, int tid, in
   if (tid == 0) { expt[kExpOne] = 1.f; expt[kExpZero] = 0.f; }
 }
 
+// Byte offset of half h of row rk in a matrix of kH halves per row: bf16
+// rows (rowb) and FP32 master rows (rowm).
+template <int kH>
+__device__ __forceinline__ unsigned rowb(int rk, int h) { return (unsigned)rk * (kH * D * 2) + (unsigned)h * (D * 2); }
+template <int kH>
+__device__ __forceinline__ unsigned rowm(int rk, int h) { return (unsigned)rk * (kH * D * 4) + (unsigned)h * (D * 4); }
+
 // Output rows of a plan for GEMM3's B operand (rows 4g .. 4g+3, elements
-// 8m .. 8m+7); rows past os read row 0 of the plan (their results are not
-// used).  A row with an FP32 master (rank < nm) is copied instead into the
-// LDS staging area (row i at stage + i * 4D bytes, straight from memory), and
-// out_rows_from_stage fills it in once the loads have been waited for.
-__device__ __forceinline__ void load_output_l3(const Mat& woh, const Mat& om, int nm, const int* plan, int os, int g,
-                                               int m, int lane, u32x4 (&o3)[4], unsigned char* stage) {
+// 8m .. 8m+7 of this wavefront's half h); rows past os read row 0 of the plan
+// (their results are not used).  A row with an FP32 master (rank < nm) is
+// copied instead into the LDS staging area (row i at stage + i * 4D bytes,
+// straight from memory), and out_rows_from_stage fills it in once the loads
+// have been waited for.
+template <int kH>
+__device__ __forceinline__ void load_output_l3(const Mat& woh, const Mat& om, int nm, const int* plan, int os, int h,
+                                               int g, int m, int lane, u32x4 (&o3)[4], unsigned char* stage) {
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     const int i = 4 * g + e;
     const int r3 = plan[i < os ? i : 0];
-    if (nm == 0 || r3 >= nm) o3[e] = woh.load((unsigned)r3 * (D * 2) + 16 * m);
+    if (nm == 0 || r3 >= nm) o3[e] = woh.load(rowb<kH>(r3, h) + 16 * m);
   }
   for (int i = 0; nm > 0 && i < os; i++) {
     const int rk = __builtin_amdgcn_readfirstlane(plan[i]);
     if (rk < nm) {
-      om.load_lds((unsigned)rk * (D * 4) + 4 * lane, stage + i * (D * 4));
-      om.load_lds((unsigned)rk * (D * 4) + 256 + 4 * lane, stage + i * (D * 4) + 256);
+      om.load_lds(rowm<kH>(rk, h) + 4 * lane, stage + i * (D * 4));
+      om.load_lds(rowm<kH>(rk, h) + 256 + 4 * lane, stage + i * (D * 4) + 256);
     }
   }
 }
@@ -645,51 +674,62 @@ __device__ __forceinline__ void out_rows_from_stage(u32x4 (&o3)[4], int nm, cons
     }
   }
 }
-// Elements 8m .. 8m+7 of FP32 master row rk.
-__device__ __forceinline__ void master_get(const Mat& M, int rk, int m, float (&e)[8]) {
-  const u32x4 lo = M.load((unsigned)rk * (D * 4) + 32 * m), hi = M.load((unsigned)rk * (D * 4) + 32 * m + 16);
+// Elements 8m .. 8m+7 of half h of FP32 master row rk.
+template <int kH>
+__device__ __forceinline__ void master_get(const Mat& M, int rk, int h, int m, float (&e)[8]) {
+  const u32x4 lo = M.load(rowm<kH>(rk, h) + 32 * m), hi = M.load(rowm<kH>(rk, h) + 32 * m + 16);
 #pragma unroll
   for (int k = 0; k < 4; k++) { e[k] = __uint_as_float(lo[k]); e[4 + k] = __uint_as_float(hi[k]); }
 }
-__device__ __forceinline__ void master_put(const Mat& M, int rk, int m, const float (&e)[8]) {
+template <int kH>
+__device__ __forceinline__ void master_put(const Mat& M, int rk, int h, int m, const float (&e)[8]) {
   u32x4 lo, hi;
 #pragma unroll
   for (int k = 0; k < 4; k++) { lo[k] = __float_as_uint(e[k]); hi[k] = __float_as_uint(e[4 + k]); }
-  M.store((unsigned)rk * (D * 4) + 32 * m, lo);
-  M.store((unsigned)rk * (D * 4) + 32 * m + 16, hi);
+  M.store(rowm<kH>(rk, h) + 32 * m, lo);
+  M.store(rowm<kH>(rk, h) + 32 * m + 16, hi);
 }
 
-// A window row goes back to Wih: its FP32 master (if it has one) exactly, or the
-// bf16 row with stochastic (or nearest) rounding.
-__device__ __forceinline__ void write_back(const Mat& wih, const Mat& wm, bool master, int rk, int m,
+// A window row (half h) goes back to Wih: its FP32 master (if it has one)
+// exactly, or the bf16 row with stochastic (or nearest) rounding.
+template <int kH>
+__device__ __forceinline__ void write_back(const Mat& wih, const Mat& wm, bool master, int rk, int h, int m,
                                            const float (&e)[8], int round_nearest, unsigned seed) {
   if (master)
-    master_put(wm, rk, m, e);
+    master_put<kH>(wm, rk, h, m, e);
   else
-    wih.store((unsigned)rk * (D * 2) + 16 * m, round_nearest ? pack8_rne(e) : pack8_sr(e, seed));
+    wih.store(rowb<kH>(rk, h) + 16 * m, round_nearest ? pack8_rne(e) : pack8_sr(e, seed));
 }
 
 // kFull: the instance with FP32 master rows, stochastic rounding of the Woh
 // updates (a.n_master, a.round_nearest) and minibatches smaller than a step
 // (a.minibatch) available; the other instance has none of them and none of
 // their code.  Both share negative groups over 2^a.share_shift steps.
-template <int WPB, bool kFull>
-__global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
+// WPB wavefronts per workgroup; kH halves per row (kH = 2: the workgroup is
+// one worker, its two wavefronts hold halves 0 and 1 of every row).
+template <int WPB, bool kFull, int kH>
+__global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
+  static_assert(kH == 1 || WPB == kH, "a row split into halves takes one workgroup per worker");
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, m = lane & 15;
   // The worker index is wave-uniform; readfirstlane lets the compiler keep it
   // (and the slice, LCG state and LDS bases derived from it) in scalar registers.
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int wk = blockIdx.x * a.wpb + wave;
+  const int h = kH > 1 ? wave % kH : 0;  // this wavefront's half of every row
+  const int wk = blockIdx.x * a.wpb + wave / kH;
   const int W = a.W, K = a.K, Lmax = a.Lmax;
   float* const expt = reinterpret_cast<float*>(smem);
   load_exp_table(expt, a.exp_table, threadIdx.x, blockDim.x);
   __syncthreads();
   if (wk >= a.workers) return;
   size_t off_unused[6];
-  const FastLds s = fast_carve(smem + exp_bytes() + (size_t)wave * fast_layout(Lmax, K, off_unused), Lmax, K);
-  const Mat wih(a.wih, (unsigned)((size_t)a.V * D * 2)), woh(a.woh, (unsigned)((size_t)a.V * D * 2));
+  const size_t wave_lds = fast_layout(Lmax, K, kFull, off_unused);
+  const FastLds s = fast_carve(smem + exp_bytes() + (size_t)wave * wave_lds, Lmax, K, kFull);
+  // kH = 2: the other half's GEMM1 products and half 0's word total
+  const FastLds s_other = fast_carve(smem + exp_bytes() + (size_t)(wave ^ (kH - 1)) * wave_lds, Lmax, K, kFull);
+  u64* const xfer = (h == 0 ? s.stats : s_other.stats) + kStatXfer;
+  const Mat wih(a.wih, (unsigned)((size_t)a.V * kH * D * 2)), woh(a.woh, (unsigned)((size_t)a.V * kH * D * 2));
   // FP32 master rows of the hottest ranks (rank < nmaster).
   const int nmaster = kFull ? a.n_master : 0;
   const int round_nearest = kFull ? a.round_nearest : 1;
@@ -701,14 +741,19 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   // Neither masters nor stochastic rounding of the Woh updates: the bf16
   // paths below take their shorter forms.
   const bool plain = !kFull || (nmaster == 0 && round_nearest);
-  const Mat wm(a.wmaster, (unsigned)((size_t)nmaster * D * 4)), om(a.omaster, (unsigned)((size_t)nmaster * D * 4));
-  // The shared negative batch of the worker: K x Lmax draws in device memory.
-  int* const negw = reinterpret_cast<int*>(a.fscratch) + (size_t)wk * fast_scratch_ints(Lmax, K);
+  const Mat wm(a.wmaster, (unsigned)((size_t)nmaster * kH * D * 4)),
+      om(a.omaster, (unsigned)((size_t)nmaster * kH * D * 4));
+  // The shared negative batch of the worker: K x Lmax draws in device memory
+  // (one copy per wavefront, so that the halves share no device memory).
+  int* const negw = reinterpret_cast<int*>(a.fscratch) + (size_t)(wk * kH + h) * fast_scratch_ints(Lmax, K);
 
   // Slot copies and coefficients start at zero (unused slots and the K
   // padding of GEMM2 must multiply as finite zeros).
   for (int x = lane; x < ((kSlots + kOutRows) * RB) / 16; x += 64) reinterpret_cast<u32x4*>(s.rc)[x] = (u32x4){0u, 0u, 0u, 0u};
-  if (lane < kSlots) { s.mult[lane] = 0; s.srank[lane] = 0; }
+  if (lane < kSlots) {
+    s.srank[lane] = 0;
+    if (kFull) s.mult[lane] = 0;
+  }
   if (lane < kStatCount) s.stats[lane] = 0;
   if (lane < 2 * 2 * kOutRows) s.plans[lane] = 0;
   typename std::conditional<kFull, SlotsQ, SlotsV>::type S;
@@ -745,7 +790,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   // sentence's current position).
   unsigned pm = 0;
   const int stride = record_stride(W, K);
-  int32_t* const rec = a.rec ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
+  int32_t* const rec = (a.rec && h == 0) ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
   int plan = 0;
   bool planned = false;
   int pre_os = 0;
@@ -756,7 +801,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   int in_rk[2] = {0, 0};    // ranks of those rows
   bool staged = false;      // FP32 master rows are on their way into LDS
   unsigned char* const ostage = s.rc + kStageOff;  // FP32 master output rows of the next step
-  unsigned salt = hash32((unsigned)wk * 0x9E3779B9u + 0x6A09E667u);
+  unsigned salt = hash32((unsigned)wk * 0x9E3779B9u + 0x6A09E667u + (unsigned)h * 0x3C6EF372u);
   int j0 = 0;      // first context of the step's current minibatch
   int b_step = 0;  // the step's window shrink, drawn at its first minibatch
 
@@ -772,8 +817,16 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     if (wc - last_wc > 10000) {
       const u64 diff = wc - last_wc;
       u64 old = 0;
-      if (lane == 0) old = atomicAdd(&a.counters[kWordCountActual], diff);
-      const u64 total = readlane64(old, 0) + diff;
+      if (lane == 0 && h == 0) old = atomicAdd(&a.counters[kWordCountActual], diff);
+      u64 total = readlane64(old, 0) + diff;
+      if constexpr (kH > 1) {
+        // both halves run this at the same point of the same index stream;
+        // half 0 passes its total to the other
+        if (lane == 0 && h == 0) *xfer = total;
+        __syncthreads();
+        total = *xfer;
+        __syncthreads();
+      }
       last_wc = wc;
       alpha = a0 * (1 - (float)total / a.alpha_den);
       if (alpha < a0 * 0.0001f) alpha = a0 * 0.0001f;
@@ -828,19 +881,25 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         len[q] = L > 1 ? L : 0;
         if (L <= 1) continue;
         wave_sync();
-        short* const sq = s.sop + q * Lmax;
+        short* const sq = kFull ? s.sop + q * Lmax : nullptr;
         // Slots of the first window [0, W]: the k-th distinct rank takes slot 24q + k.
         const int nw = min(W, L - 1) + 1;
-        if constexpr (kFull) occurrences16(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, nw, lane);
-        else occurrences_near(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, nw, 2 * W + 1, lane);
+        int first = 0;  // lane p < nw: first position of p's rank
+        if constexpr (kFull) {
+          occurrences16(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, nw, lane);
+          wave_sync();
+          if (lane < nw) first = sq[lane];
+        } else {
+          first = occurrences_near(tq, s.prv + q * Lmax, L, nw, 2 * W + 1, lane);
+        }
         wave_sync();
         const bool isnew = lane < nw && s.prv[q * Lmax + lane] < 0;
         const u64 newmask = __ballot(isnew);
         const int nnew = __popcll(newmask);
         int myslot = 0;
-        if (lane < nw) myslot = kSentSlots * q + __popcll(newmask & lanemask_lt(sq[lane]));
+        if (lane < nw) myslot = kSentSlots * q + __popcll(newmask & lanemask_lt(first));
         wave_sync();
-        if (lane < nw) sq[lane] = (short)myslot;
+        if (kFull && lane < nw) sq[lane] = (short)myslot;
         if constexpr (!kFull) {
           pm = (lane >= kSentSlots * q && lane < kSentSlots * (q + 1)) ? 0u : pm;
           for (int p = 0; p < nw; p++) {
@@ -859,7 +918,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         unsigned char* const stg = kFull ? s.rc : s.ost;
         const int krk = (lane < nnew) ? s.srank[kSentSlots * q + lane] : 0;  // rank of new row `lane`
         for (int k = 0; k < nnew; k++)
-          wih.load_lds((unsigned)__builtin_amdgcn_readlane(krk, k) * (D * 2) + 4 * lane, stg + k * (D * 2));
+          wih.load_lds(rowb<kH>(__builtin_amdgcn_readlane(krk, k), h) + 4 * lane, stg + k * (D * 2));
         wait_vm_all();
         if constexpr (!kFull) {
           // All new rows at once: each lane fills those of its slot
@@ -888,7 +947,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
           const int rk = __builtin_amdgcn_readfirstlane(s.srank[kSentSlots * q + k]);
           if (is_master(rk)) {
             float e[8];
-            master_get(wm, rk, m, e);
+            master_get<kH>(wm, rk, h, m, e);
             slot_put_f(S, kSentSlots * q + k, e, g);
           } else {
             slot_put(S, kSentSlots * q + k, *reinterpret_cast<const u32x4*>(stg + k * (D * 2) + 16 * m), g);
@@ -901,7 +960,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 
     // End of the slice: in-flight sentences are dropped (dsgl.hpp:1031-1050).
     if (cur >= end) {
-      if (lane == 0) atomicAdd(&a.counters[kWordCountActual], wc - last_wc);
+      if (lane == 0 && h == 0) atomicAdd(&a.counters[kWordCountActual], wc - last_wc);
       if (lane == 0) stat_add(s.stats, kStatRaw, wc);
       if (--iters_left == 0) break;
       wc = 0; cur = start; last_wc = 0;
@@ -968,8 +1027,12 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       const int at = k == 0 ? pq : (k == 1 ? pq + 1 : (k == 2 ? pq - W : pq + W + 1));
       gr = s.tok[q * Lmax + min(max(at, 0), last_pos)];
       const int xs = min(max(pq - W, 0), last_pos), ys = min(pq + W + 1, last_pos);
-      const short* const arr = k == 0 ? s.prv : (k == 1 ? s.nxt : s.sop);
-      gs = arr[q * Lmax + (k == 0 ? ys : xs)];
+      if constexpr (kFull) {
+        const short* const arr = k == 0 ? s.prv : (k == 1 ? s.nxt : s.sop);
+        gs = arr[q * Lmax + (k == 0 ? ys : xs)];
+      } else {
+        gs = s.prv[q * Lmax + ys];  // the plain instance reads lanes 0 and 1 only
+      }
     }
     const int t1 = __builtin_amdgcn_readlane(gr, 0);
     const int t2 = __builtin_amdgcn_readlane(gr, 1);
@@ -1009,9 +1072,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     // positions within the step's window, minus the centre (positions
     // outside the sentence are never set).
     int my_ctx_rank = -1;
+    int cnt = 0;  // lane s < 48: the step's contexts in slot s
     if constexpr (!kFull) {
       const unsigned ctxm = ((((2u << (2 * hw)) - 1u) << (W - hw)) & ~(1u << W));
-      if (lane < kSlots) s.mult[lane] = __builtin_popcount(pm & ctxm);
+      cnt = __builtin_popcount(pm & ctxm);  // 0 in lanes >= 48 (pm = 0 there)
       if (rec && lane < nin) {
         const int q = (lane < n0) ? 0 : 1;
         const int jj = q ? lane - n0 : lane;
@@ -1067,7 +1131,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       os = pre_os;
     } else {
       os = make_outputs(plan, t1, t2, (lane < K) ? negw[grp * K + lane] : -1);
-      load_output_l3(woh, om, nmaster, s.plans + plan * 2 * kOutRows, os, g, m, lane, o3, ostage);
+      load_output_l3<kH>(woh, om, nmaster, s.plans + plan * 2 * kOutRows, os, h, g, m, lane, o3, ostage);
       if (nmaster > 0) {
         wait_vm_all();
         out_rows_from_stage(o3, nmaster, s.plans + plan * 2 * kOutRows, os, g, m, ostage);
@@ -1123,9 +1187,10 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       meta4[0] = mm.x; meta4[1] = mm.y; meta4[2] = mm.z; meta4[3] = mm.w;
     }
     wave_sync();
-    const StepLds L{s.rc, s.cb, s.mult};
+    if constexpr (kFull) cnt = (lane < kSlots) ? s.mult[lane] : 0;
+    const StepLds L{s.rc, s.cb};
     float C[3][4];
-    gemm1_coef(s.ost, L, meta4, os, alpha, expt, g, m, C);
+    gemm1_coef<kH>(s.ost, L, meta4, os, alpha, expt, g, m, cnt, C, s.rc + kMdOff, s_other.rc + kMdOff);
     wave_sync();
     {
       // md in FP32, staged row by row in LDS (rc is free until the next
@@ -1175,7 +1240,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 #pragma unroll
           for (int i = 0; i < NR; i++)
             if (i < os) {
-              const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (D / 2);
+              const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (kH * D / 2) + h * (D / 2);
               (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + lane, __builtin_bit_cast(short2v, v[i]));
             }
         };
@@ -1189,8 +1254,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         if (is_master(rk)) {
           const float v0 = *reinterpret_cast<const float*>(mdl + i * MDS + 4 * lane);
           const float v1 = *reinterpret_cast<const float*>(mdl + i * MDS + 256 + 4 * lane);
-          (void)__builtin_amdgcn_global_atomic_fadd_f32(ob + (size_t)rk * D + lane, v0);
-          (void)__builtin_amdgcn_global_atomic_fadd_f32(ob + (size_t)rk * D + 64 + lane, v1);
+          (void)__builtin_amdgcn_global_atomic_fadd_f32(ob + (size_t)rk * (kH * D) + h * D + lane, v0);
+          (void)__builtin_amdgcn_global_atomic_fadd_f32(ob + (size_t)rk * (kH * D) + h * D + 64 + lane, v1);
         } else {
           const float2 mv = *reinterpret_cast<const float2*>(mdl + i * MDS + 8 * lane);
           unsigned u;
@@ -1210,7 +1275,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
             const unsigned tb = __builtin_bit_cast(unsigned, t);
             u = pack_rne(__uint_as_float(tb << 16) - o0, __uint_as_float(tb & 0xFFFF0000u) - o1);
           }
-          (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + (unsigned)rk * (D / 2) + lane, __builtin_bit_cast(short2v, u));
+          (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + (unsigned)rk * (kH * D / 2) + h * (D / 2) + lane,
+                                                           __builtin_bit_cast(short2v, u));
         }
       }
     }
@@ -1224,7 +1290,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       const int rk = (last && in_slot_src[q] == -1) ? __builtin_amdgcn_readlane(gr, 6 + q) : -1;
       pend_hot[q] = rk >= 0 && is_master(rk);  // staged into ost after GEMM3 below
       in_rk[q] = rk;
-      if (!pend_hot[q]) in_row[q] = wih.load((unsigned)(rk >= 0 ? rk : 0) * (D * 2) + 16 * m);
+      if (!pend_hot[q]) in_row[q] = wih.load(rowb<kH>(rk >= 0 ? rk : 0, h) + 16 * m);
     }
     // Plan the next step's outputs and read its rows (after this step's
     // updates of the same rows), when neither sentence ends here.
@@ -1233,7 +1299,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     if (cont) {
       const int np = plan ^ 1;
       osn = make_outputs(np, __builtin_amdgcn_readlane(gr, 2), __builtin_amdgcn_readlane(gr, 3), (lane < K) ? vnext : -1);
-      load_output_l3(woh, om, nmaster, s.plans + np * 2 * kOutRows, osn, g, m, lane, o3, ostage);
+      load_output_l3<kH>(woh, om, nmaster, s.plans + np * 2 * kOutRows, osn, h, g, m, lane, o3, ostage);
       staged = nmaster > 0;
     } else {
       // the next step plans afresh: the rows are dead (and need no register
@@ -1254,8 +1320,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
 #pragma unroll
     for (int q = 0; q < 2; q++)
       if (pend_hot[q]) {
-        wm.load_lds((unsigned)in_rk[q] * (D * 4) + 4 * lane, s.ost + q * (D * 4));
-        wm.load_lds((unsigned)in_rk[q] * (D * 4) + 256 + 4 * lane, s.ost + q * (D * 4) + 256);
+        wm.load_lds(rowm<kH>(in_rk[q], h) + 4 * lane, s.ost + q * (D * 4));
+        wm.load_lds(rowm<kH>(in_rk[q], h) + 256 + 4 * lane, s.ost + q * (D * 4) + 256);
         staged = true;
       }
 
@@ -1286,7 +1352,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
       const int p = pos[q];
       const int Lq = len[q];
       int* const tq = s.tok + q * Lmax;
-      short* const sq = s.sop + q * Lmax;
+      short* const sq = kFull ? s.sop + q * Lmax : nullptr;
       pos[q] = p + 1;
       if (p + 1 >= Lq) {
         if (!kFull) {
@@ -1301,7 +1367,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
             }
             if (sl4 >= 0) {
               const int rk = s.srank[sl4];
-              wih.store((unsigned)rk * (D * 2) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sl4, m)));
+              wih.store(rowb<kH>(rk, h) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sl4, m)));
             }
           }
         } else
@@ -1311,7 +1377,8 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
           slot_get(S, sl, e);
           const int rk = __builtin_amdgcn_readfirstlane(s.srank[sl]);
           if (g == ((sl >> 2) & 3))
-            write_back(wih, wm, is_master(rk), rk, m, e, round_nearest, salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sl << 24));
+            write_back<kH>(wih, wm, is_master(rk), rk, h, m, e, round_nearest,
+                           salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sl << 24));
         }
         freem[q] = 0xFFFFFFu;
         len[q] = 0;
@@ -1335,7 +1402,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         }
         if (x >= 0 && __builtin_amdgcn_readlane(pm, sx) == 0) {
           const int rkx = __builtin_amdgcn_readlane(gr, 4 + q);
-          if (g == q) wih.store((unsigned)rkx * (D * 2) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
+          if (g == q) wih.store(rowb<kH>(rkx, h) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
           freem[q] |= 1u << (sx - kSentSlots * q);
         }
         if (in_slot_src[q] == -1) {
@@ -1354,15 +1421,13 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
         const int sx = __builtin_amdgcn_readlane(gs, 4 + q);
         const int rkx = __builtin_amdgcn_readlane(gr, 4 + q);
         if (!kFull) {
-          if (g == q) wih.store((unsigned)rkx * (D * 2) + 16 * m,
-                                *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
+          if (g == q) wih.store(rowb<kH>(rkx, h) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
         } else {
         float e[8];
         slot_get(S, sx, e);
         if (g == ((sx >> 2) & 3))
-          write_back(wih, wm, is_master(rkx), rkx, m, e,
-                     round_nearest,
-                     salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sx << 24));
+          write_back<kH>(wih, wm, is_master(rkx), rkx, h, m, e, round_nearest,
+                         salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sx << 24));
         }
         freem[q] |= 1u << (sx - kSentSlots * q);
       }
@@ -1388,7 +1453,7 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
     wave_sync();
   }
 
-  if (lane == 0) {
+  if (lane == 0 && h == 0) {
     a.worker_steps[wk] = steps;
     a.worker_alpha[wk] = alpha;
     atomicAdd(&a.counters[kSteps], steps);
@@ -1399,16 +1464,22 @@ __global__ __launch_bounds__(64 * WPB) void k_train_fast(TrainArgs a) {
   }
 }
 
-// One step of the window-slot form on given rows (test-facing): O (os x D
-// float), slot rows I (48 x D float, slots 0..23 first sentence), context
+// One step of the window-slot form on given rows (test-facing): O (os x kH D
+// float), slot rows I (48 x kH D float, slots 0..23 first sentence), context
 // count per slot, output multiplicities.  Writes the coefficients (os x 48),
-// md (os x D) and the updated slot rows (48 x D), each through the same
-// device code as the trainer.
-__global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha,
-                                                  const float* exp_table, const float* O, const float* I,
-                                                  float* corr, float* md_out, float* I_out) {
-  __shared__ __align__(16) unsigned char lds[(kSlots + kOutRows) * RB + 2 * kSlots * 4 + 4096 + 64];
-  const int lane = threadIdx.x, g = lane >> 4, m = lane & 15;
+// md (os x kH D) and the updated slot rows (48 x kH D), each through the same
+// device code as the trainer; wavefront h of the workgroup takes half h of
+// every row.
+template <int kH>
+__global__ __launch_bounds__(64 * kH) void k_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha,
+                                                       const float* exp_table, const float* O, const float* I,
+                                                       float* corr, float* md_out, float* I_out) {
+  constexpr int kWaveLds = (kSlots + kOutRows) * RB + 2 * kSlots * 4 + 4096 + 64;
+  __shared__ __align__(16) unsigned char lds_all[kH * kWaveLds];
+  const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
+  const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int DS = kH * D;  // row length of O, I, md, I_out
+  unsigned char* const lds = lds_all + h * kWaveLds;
   unsigned char* rc = lds;
   unsigned char* ost = lds + kSlots * RB;
   unsigned char* cb = rc;
@@ -1424,12 +1495,12 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) S.T[rt][4 * (t) + (r)] = I[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t];
+      for (int r = 0; r < 4; r++) S.T[rt][4 * (t) + (r)] = I[(size_t)(16 * rt + 4 * g + r) * DS + h * D + 8 * m + t];
   // Output rows as bf16 in LDS (rows past os repeat row 0).
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     const int i = 4 * g + e;
-    const float* src = O + (size_t)(i < os ? i : 0) * D + 8 * m;
+    const float* src = O + (size_t)(i < os ? i : 0) * DS + h * D + 8 * m;
     u32x4 o;
 #pragma unroll
     for (int k = 0; k < 4; k++) o[k] = pack_rne(src[2 * k], src[2 * k + 1]);
@@ -1441,26 +1512,30 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
 #pragma unroll
   for (int r = 0; r < 4; r++) meta4[r] = plan[kOutRows + 4 * g + r];
   __syncthreads();
-  const StepLds L{rc, cb, ml};
+  const StepLds L{rc, cb};
+  const int cnt = (lane < kSlots) ? ml[lane] : 0;
   float C[3][4];
-  gemm1_coef(ost, L, meta4, os, alpha, expt, g, m, C);
+  gemm1_coef<kH>(ost, L, meta4, os, alpha, expt, g, m, cnt, C, rc + kMdOff,
+                 lds_all + (h ^ (kH - 1)) * kWaveLds + kMdOff);
+  if (h == 0)
 #pragma unroll
-  for (int rt = 0; rt < 3; rt++)
+    for (int rt = 0; rt < 3; rt++)
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-      if (4 * g + r < os) corr[(size_t)(4 * g + r) * kSlots + 16 * rt + m] = C[rt][r];
+      for (int r = 0; r < 4; r++)
+        if (4 * g + r < os) corr[(size_t)(4 * g + r) * kSlots + 16 * rt + m] = C[rt][r];
   __syncthreads();
   const bf16x8 c0 = *reinterpret_cast<const bf16x8*>(cb + m * CBS + g * 32);
   const bf16x8 c1 = *reinterpret_cast<const bf16x8*>(cb + m * CBS + g * 32 + 16);
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
+  for (int hh = 0; hh < 2; hh++) {
     f32x4 md[NT / 2];
-    gemm2_half(S, c0, c1, h * (NT / 2), md);
+    gemm2_half(S, c0, c1, hh * (NT / 2), md);
 #pragma unroll
     for (int r = 0; r < 4; r++)
       if (4 * g + r < os)
 #pragma unroll
-        for (int t = 0; t < NT / 2; t++) md_out[(size_t)(4 * g + r) * D + 8 * m + h * (NT / 2) + t] = md[t][r];
+        for (int t = 0; t < NT / 2; t++)
+          md_out[(size_t)(4 * g + r) * DS + h * D + 8 * m + hh * (NT / 2) + t] = md[t][r];
   }
   gemm3(S, C, ost, g, m);
 #pragma unroll
@@ -1468,55 +1543,84 @@ __global__ __launch_bounds__(64) void k_step_fast(int os, const int32_t* meta, c
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) I_out[(size_t)(16 * rt + 4 * g + r) * D + 8 * m + t] = S.T[rt][4 * (t) + (r)];
+      for (int r = 0; r < 4; r++)
+        I_out[(size_t)(16 * rt + 4 * g + r) * DS + h * D + 8 * m + t] = S.T[rt][4 * (t) + (r)];
 }
 
 }  // namespace
 
 bool fast_supported(int D_, int K, int W, uint32_t V, int64_t table_size) {
-  return D_ == D && K + 2 <= kOutRows && 2 * W + 1 <= kSentSlots && W >= 1 && V >= 2 &&
-         (uint64_t)V * D * 2 < (1ull << 31) && table_size >= 1 && table_size < (1ll << 31);
+  const int nh = fast_halves(D_);
+  return nh > 0 && K + 2 <= kOutRows && 2 * W + 1 <= kSentSlots && W >= 1 && V >= 2 &&
+         (uint64_t)V * (uint64_t)D_ * 2 < (1ull << 31) && table_size >= 1 && table_size < (1ll << 31);
 }
 
-size_t fast_lds_bytes(int Lmax, int K) {
+size_t fast_lds_bytes(int Lmax, int K, bool full) {
   size_t off[6];
-  return fast_layout(Lmax, K, off);
+  return fast_layout(Lmax, K, full, off);
 }
 
-size_t fast_block_lds_bytes(int Lmax, int K, int wpb) { return exp_bytes() + (size_t)wpb * fast_lds_bytes(Lmax, K); }
+size_t fast_block_lds_bytes(int Lmax, int K, int wpb, int D_, bool full) {
+  return exp_bytes() + (size_t)wpb * (size_t)fast_halves(D_) * fast_lds_bytes(Lmax, K, full);
+}
 
-int fast_workers_per_block(int Lmax, int K, size_t lds_per_block) {
-  const size_t per = fast_lds_bytes(Lmax, K);
+int fast_workers_per_block(int Lmax, int K, size_t lds_per_block, int D_) {
+  const int nh = fast_halves(D_);
+  if (nh < 1) return 0;
+  // sized for the full instance's layout (the larger), so that the count
+  // fits whichever instance a run takes
+  const size_t per = (size_t)nh * fast_lds_bytes(Lmax, K, true);
   if (lds_per_block < exp_bytes() + per) return 0;
+  if (nh > 1) return 1;  // the workgroup's barriers pair the halves of one worker
   const size_t n = (lds_per_block - exp_bytes()) / per;
   return (int)(n < (size_t)kFastMaxWpb ? n : (size_t)kFastMaxWpb);
 }
 
+namespace {
+// The trainer instance for a run: masters, stochastic rounding or
+// minibatches take the full instance; halves per row from D.
+template <bool kFull, int kH>
+const void* train_fast_fn() {
+  return reinterpret_cast<const void*>(&k_train_fast<kH == 1 ? kFastMaxWpb : kH, kFull, kH>);
+}
+bool fast_full(const TrainArgs& a) { return a.n_master > 0 || !a.round_nearest || a.minibatch > 0; }
+const void* pick_train_fast(const TrainArgs& a) {
+  const bool full = fast_full(a);
+  if (fast_halves(a.D) == 2) return full ? train_fast_fn<true, 2>() : train_fast_fn<false, 2>();
+  return full ? train_fast_fn<true, 1>() : train_fast_fn<false, 1>();
+}
+bool fast_args_ok(const TrainArgs& a) {
+  const int nh = fast_halves(a.D);
+  return nh > 0 && a.wpb >= 1 && a.wpb <= (nh == 1 ? kFastMaxWpb : 1);
+}
+}  // namespace
+
 hipError_t prepare_train_fast(const TrainArgs& a) {
-  if (a.wpb < 1 || a.wpb > kFastMaxWpb) return hipErrorInvalidValue;
-  const int lds = (int)fast_block_lds_bytes(a.Lmax, a.K, a.wpb);
-  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train_fast<kFastMaxWpb, false>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train_fast<kFastMaxWpb, true>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (!fast_args_ok(a)) return hipErrorInvalidValue;
+  const int lds = (int)fast_block_lds_bytes(a.Lmax, a.K, a.wpb, a.D, fast_full(a));
+  return hipFuncSetAttribute(pick_train_fast(a), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 
 hipError_t launch_train_fast(const TrainArgs& a, hipStream_t st) {
-  if (a.wpb < 1 || a.wpb > kFastMaxWpb) return hipErrorInvalidValue;
-  const size_t lds = fast_block_lds_bytes(a.Lmax, a.K, a.wpb);
-  if (a.n_master > 0 || !a.round_nearest || a.minibatch > 0)
-    hipLaunchKernelGGL((k_train_fast<kFastMaxWpb, true>), dim3((a.workers + a.wpb - 1) / a.wpb), dim3(64 * a.wpb), lds,
-                       st, a);
-  else
-    hipLaunchKernelGGL((k_train_fast<kFastMaxWpb, false>), dim3((a.workers + a.wpb - 1) / a.wpb), dim3(64 * a.wpb), lds,
-                       st, a);
-  return hipGetLastError();
+  if (!fast_args_ok(a)) return hipErrorInvalidValue;
+  const size_t lds = fast_block_lds_bytes(a.Lmax, a.K, a.wpb, a.D, fast_full(a));
+  const dim3 grid((a.workers + a.wpb - 1) / a.wpb), block(64 * a.wpb * fast_halves(a.D));
+  TrainArgs args = a;
+  void* params[] = {&args};
+  return hipLaunchKernel(pick_train_fast(a), grid, block, params, lds, st);
 }
 
-hipError_t launch_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha, const float* exp_table,
-                            const float* O, const float* I, float* corr, float* md, float* I_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_fast, dim3(1), dim3(64), 0, st, os, meta, mult, alpha, exp_table, O, I, corr, md, I_out);
+hipError_t launch_step_fast(int D_, int os, const int32_t* meta, const int32_t* mult, float alpha,
+                            const float* exp_table, const float* O, const float* I, float* corr, float* md,
+                            float* I_out, hipStream_t st) {
+  if (fast_halves(D_) == 2)
+    hipLaunchKernelGGL(k_step_fast<2>, dim3(1), dim3(128), 0, st, os, meta, mult, alpha, exp_table, O, I, corr, md,
+                       I_out);
+  else if (fast_halves(D_) == 1)
+    hipLaunchKernelGGL(k_step_fast<1>, dim3(1), dim3(64), 0, st, os, meta, mult, alpha, exp_table, O, I, corr, md,
+                       I_out);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -417,7 +417,7 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   if ((c->share != 2 || c->minibatch != 0 || opts->autotune == 2) && !c->fast)
     return bail(fail(DGER_E_UNSUPPORTED,
                      "-share, -batch-parity 0 and -autotune 2 need the window-slot trainer "
-                     "(-precision bf16-mfma, -size 128, -window <= 11, -negative <= 14)"));
+                     "(-precision bf16-mfma, -size 128 or 256, -window <= 11, -negative <= 14)"));
   // The allocation-free entry point's scratch address on this device is
   // looked up once here (dger_ctx_train_noalloc reports a failure).
   if (c->fast) {
@@ -565,7 +565,7 @@ static int auto_workers(dger_ctx* c, size_t block_lds, int wpb) {
 }
 
 static int ensure_worker_buffers(dger_ctx* c, int workers) {
-  const size_t per = c->fast ? dger::fast_scratch_ints(c->Lmax, c->o.negative)
+  const size_t per = c->fast ? dger::fast_worker_scratch_ints(c->Lmax, c->o.negative, c->D)
                              : dger::worker_scratch_floats(c->Lmax, c->o.negative, c->D);
   if (c->scratch_d.ensure((size_t)workers * per * 4) || c->wsteps_d.ensure((size_t)workers * 8) ||
       c->walpha_d.ensure((size_t)workers * 4))
@@ -671,12 +671,14 @@ static int run_pass(dger_ctx* c, const dger::TrainArgs& a, bool use_graph, float
 
 // Autotuner: times one pass over a corpus prefix per candidate and keeps the
 // fastest in trained words per second.  Level 1: one to four workers per CU
-// (eight for the window-slot trainer), one workgroup per CU.  Level 2 (the
+// (eight for the window-slot trainer at size 128, one workgroup per CU; at
+// size 256 one worker per workgroup, up to the workgroups the CU's LDS
+// holds).  Level 2 (the
 // window-slot trainer) then also times, at that worker count, negative
 // sharing 2, 4, 8 with and without -batch-size 2*window+1 minibatches; these
 // change the training, so the caller gates the choice on embedding quality.
 // The model is saved before and restored after.
-static int autotune_run(dger_ctx* c, int cus, int wpb_max, int* out) {
+static int autotune_run(dger_ctx* c, int cus, int per_cu_max, int wpb_max, int* out) {
   DevBuf* model[4] = {&c->wih_d, &c->woh_d, &c->wm_d, &c->om_d};
   DevBuf save[4];
   for (int m = 0; m < 4; m++) {
@@ -697,15 +699,16 @@ static int autotune_run(dger_ctx* c, int cus, int wpb_max, int* out) {
   };
   double best = -1.0;
   int best_w = 0, best_wpb = 1;
-  for (int per_cu = 1; per_cu <= std::min(c->fast ? dger::kFastMaxWpb : 4, wpb_max); per_cu++) {
+  for (int per_cu = 1; per_cu <= per_cu_max; per_cu++) {
     const int w = std::max<int>(1, std::min<uint64_t>((uint64_t)cus * per_cu, std::max<uint64_t>(1, sample / 4096)));
+    const int wpb = std::min(std::min(per_cu, wpb_max), w);
     double rate = 0.0;
-    if (const int rc = time_pass(w, std::min(per_cu, w), &rate); rc != DGER_OK) return rc;
+    if (const int rc = time_pass(w, wpb, &rate); rc != DGER_OK) return rc;
     if (c->o.debug > 2) fprintf(stderr, "dger: autotune %d workers: %.1f M words/s\n", w, rate * 1e-3);
     if (rate > best) {
       best = rate;
       best_w = w;
-      best_wpb = std::min(per_cu, w);
+      best_wpb = wpb;
     }
   }
   if (c->o.autotune >= 2 && c->fast) {
@@ -746,10 +749,11 @@ static int pick_workers(dger_ctx* c, int32_t workers, bool allow_autotune, int* 
   // Workgroups may take more LDS than the default per-block figure by opting in.
   const size_t lds_block_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
   auto block_lds = [&](int n) {
-    return c->fast ? dger::fast_block_lds_bytes(c->Lmax, c->o.negative, n)
+    return c->fast ? dger::fast_block_lds_bytes(c->Lmax, c->o.negative, n, c->D,
+                                                c->n_master > 0 || c->o.rounding != 1 || c->minibatch > 0)
                    : dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision, n);
   };
-  int wpb = c->fast ? dger::fast_workers_per_block(c->Lmax, c->o.negative, lds_block_max)
+  int wpb = c->fast ? dger::fast_workers_per_block(c->Lmax, c->o.negative, lds_block_max, c->D)
                     : dger::train_workers_per_block(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision,
                                                     lds_block_max);
   if (wpb < 1)
@@ -761,7 +765,13 @@ static int pick_workers(dger_ctx* c, int32_t workers, bool allow_autotune, int* 
   if (workers <= 0) workers = c->o.workers;
   if (workers <= 0 && c->tuned_workers > 0) workers = c->tuned_workers;
   if (workers <= 0 && c->o.autotune && allow_autotune) {
-    const int rc = autotune_run(c, prop.multiProcessorCount, wpb_max, &workers);
+    const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                                : (size_t)160 * 1024;
+    const int per_cu_max =
+        (c->fast && dger::fast_halves(c->D) > 1)
+            ? (int)std::max<size_t>(1, lds_cu / block_lds(1))
+            : std::min(c->fast ? dger::kFastMaxWpb : 4, wpb_max);
+    const int rc = autotune_run(c, prop.multiProcessorCount, per_cu_max, wpb_max, &workers);
     if (rc != DGER_OK) return rc;
     c->tuned_workers = workers;
   }
@@ -839,7 +849,7 @@ static size_t noalloc_layout(dger_ctx* c, int workers, size_t* off) {
   off[1] = al256((size_t)dger::kNumCounters * 8);
   off[2] = off[1] + al256((size_t)workers * 8);
   off[3] = off[2] + al256((size_t)workers * 4);
-  return off[3] + (size_t)workers * dger::fast_scratch_ints(c->Lmax, c->o.negative) * 4;
+  return off[3] + (size_t)workers * dger::fast_worker_scratch_ints(c->Lmax, c->o.negative, c->D) * 4;
 }
 
 int dger_ctx_train_noalloc(dger_ctx* c, int32_t workers, dger_sgns_stats* stats) {
@@ -848,7 +858,7 @@ int dger_ctx_train_noalloc(dger_ctx* c, int32_t workers, dger_sgns_stats* stats)
   if (!c->fast)
     return fail(DGER_E_UNSUPPORTED,
                 "dger_ctx_train_noalloc runs the window-slot trainer only (DGER_PRECISION_BF16_MFMA, "
-                "size 128, window <= 11, negative <= 14); use dger_ctx_train");
+                "size 128 or 256, window <= 11, negative <= 14); use dger_ctx_train");
   DGER_HIP(hipSetDevice(c->o.device));
   void* sp = nullptr;
   DGER_HIP(dger::noalloc_scratch(c->o.device, &sp));
@@ -1087,7 +1097,13 @@ int dger_sgns_step_mfma_gemm1(int32_t D, int32_t n_out, int32_t n_ctx, int32_t n
 
 int dger_sgns_step_fast(int32_t n_out, const int32_t* meta, const int32_t* mult, float alpha, const float* O,
                         const float* I, float* corr, float* md, float* I_out) {
-  const int D = dger::kFastDim, S = 48;
+  return dger_sgns_step_fast_d(dger::kFastDim, n_out, meta, mult, alpha, O, I, corr, md, I_out);
+}
+
+int dger_sgns_step_fast_d(int32_t size, int32_t n_out, const int32_t* meta, const int32_t* mult, float alpha,
+                          const float* O, const float* I, float* corr, float* md, float* I_out) {
+  const int D = size, S = 48;
+  if (dger::fast_halves(D) < 1) return fail(DGER_E_ARG, "window-slot step: size must be 128 or 256");
   if (n_out < 2 || n_out > 16 || !meta || !mult || !O || !I || !corr || !md || !I_out)
     return fail(DGER_E_ARG, "bad window-slot step arguments");
   float exp_table[dger::kExpTable];
@@ -1102,7 +1118,7 @@ int dger_sgns_step_fast(int32_t n_out, const int32_t* meta, const int32_t* mult,
   DGER_HIP(hipMemcpy(de.p, exp_table, sizeof(exp_table), hipMemcpyHostToDevice));
   DGER_HIP(hipMemcpy(dO.p, O, (size_t)n_out * D * 4, hipMemcpyHostToDevice));
   DGER_HIP(hipMemcpy(dI.p, I, (size_t)S * D * 4, hipMemcpyHostToDevice));
-  DGER_HIP(dger::launch_step_fast(n_out, dm.as<int32_t>(), dmu.as<int32_t>(), alpha, de.as<float>(), dO.as<float>(),
+  DGER_HIP(dger::launch_step_fast(D, n_out, dm.as<int32_t>(), dmu.as<int32_t>(), alpha, de.as<float>(), dO.as<float>(),
                                   dI.as<float>(), dc.as<float>(), dmd.as<float>(), dI2.as<float>(), nullptr));
   DGER_HIP(hipDeviceSynchronize());
   DGER_HIP(hipMemcpy(corr, dc.p, (size_t)n_out * S * 4, hipMemcpyDeviceToHost));

@@ -91,16 +91,28 @@ size_t train_block_lds_bytes(int Lmax, int K, int W, int D, int precision, int w
 DGER_HD inline int record_stride(int W, int K) { return 8 + 4 * W + 2 * (K + 2); }
 
 // Window-slot trainer of the BF16-MFMA form (sgns_fast.hip): bf16 rows of
-// length kFastDim, window slots in registers, contractions on MFMA.
+// length kFastDim per wavefront (one wavefront per worker at D = 128, two at
+// D = 256, each holding one half of every row), window slots in registers,
+// contractions on MFMA.
 constexpr int kFastDim = 128;
-constexpr int kFastMaxWpb = 8;  // workers (wavefronts) per workgroup: two per SIMD
+constexpr int kFastMaxWpb = 8;  // workers (wavefronts) per workgroup at D = 128: two per SIMD
+// Wavefronts per worker (halves of a row) for row length D; 0: not supported.
+DGER_HD inline int fast_halves(int D) { return D == kFastDim ? 1 : (D == 2 * kFastDim ? 2 : 0); }
 bool fast_supported(int D, int K, int W, uint32_t V, int64_t table_size);
-size_t fast_lds_bytes(int Lmax, int K);
-// Device memory per worker of the window-slot trainer, in 4-byte words: its
-// shared negative batch (K x Lmax ranks).
+// One wavefront's LDS; `full`: the instance with masters, stochastic
+// rounding or minibatches (n_master > 0, round_nearest == 0 or minibatch > 0).
+size_t fast_lds_bytes(int Lmax, int K, bool full);
+// Device memory per wavefront of the window-slot trainer, in 4-byte words:
+// its copy of the worker's shared negative batch (K x Lmax ranks).
 DGER_HD inline size_t fast_scratch_ints(int Lmax, int K) { return (size_t)(K > 0 ? K : 1) * (size_t)Lmax; }
-size_t fast_block_lds_bytes(int Lmax, int K, int wpb);
-int fast_workers_per_block(int Lmax, int K, size_t lds_per_block);
+// The same per worker (all its wavefronts).
+DGER_HD inline size_t fast_worker_scratch_ints(int Lmax, int K, int D) {
+  return (size_t)fast_halves(D) * fast_scratch_ints(Lmax, K);
+}
+// LDS of a workgroup of wpb workers; workers per workgroup that fit
+// lds_per_block in either instance (0 if not even one; always 1 at D = 256).
+size_t fast_block_lds_bytes(int Lmax, int K, int wpb, int D, bool full);
+int fast_workers_per_block(int Lmax, int K, size_t lds_per_block, int D);
 hipError_t prepare_train_fast(const TrainArgs& a);
 // Device scratch of the allocation-free trainer entry point: a static device
 // array of kNoAllocScratchBytes per GPU.  noalloc_scratch returns this
@@ -108,9 +120,10 @@ hipError_t prepare_train_fast(const TrainArgs& a);
 constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
 hipError_t noalloc_scratch(int device, void** ptr);
 hipError_t launch_train_fast(const TrainArgs& a, hipStream_t s);
-// One step of the window-slot form on host-staged rows (test-facing).
-hipError_t launch_step_fast(int os, const int32_t* meta, const int32_t* mult, float alpha, const float* exp_table,
-                            const float* O, const float* I, float* corr, float* md, float* I_out, hipStream_t s);
+// One step of the window-slot form on host-staged rows of length D (test-facing).
+hipError_t launch_step_fast(int D, int os, const int32_t* meta, const int32_t* mult, float alpha,
+                            const float* exp_table, const float* O, const float* I, float* corr, float* md,
+                            float* I_out, hipStream_t s);
 
 hipError_t launch_init_net(float* wih, uint64_t n, int D, hipStream_t s);
 hipError_t launch_table_fill(int32_t* table, uint64_t table_size, const int64_t* starts, int n_starts,

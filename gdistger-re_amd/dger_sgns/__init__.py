@@ -119,6 +119,7 @@ _SIGS = {
     "dger_ctx_destroy": (None, [_vp]),
     "dger_sgns_step": (C.c_int, [_i32, _i32, _i32, _i32, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp]),
     "dger_sgns_step_fast": (C.c_int, [_i32, _vp, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp]),
+    "dger_sgns_step_fast_d": (C.c_int, [_i32, _i32, _vp, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp]),
     "dger_loss_fwd_f32": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp]),
     "dger_loss_bwd_f32": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, C.c_float, _vp, _vp]),
     "dger_loss_fwd_f64": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp]),
@@ -503,26 +504,27 @@ def step_mfma_gemm1(O, I, fin: int, meta, alpha: float) -> np.ndarray:
 
 
 FAST_DIM = 128
+FAST_DIMS = (128, 256)  # row lengths of the window-slot trainer
 FAST_SLOTS = 48
 
 
 def step_fast(O, I, mult, meta, alpha: float):
     """One step of the window-slot BF16-MFMA trainer on the device
-    (dger_sgns_step_fast): O (n_out x 128), I (48 x 128 slot rows), mult (48
-    contexts per slot), meta (n_out).  Returns corr (n_out x 48), md
-    (n_out x 128) and the updated slot rows (48 x 128)."""
+    (dger_sgns_step_fast_d): O (n_out x D), I (48 x D slot rows), mult (48
+    contexts per slot), meta (n_out), D = 128 or 256.  Returns corr
+    (n_out x 48), md (n_out x D) and the updated slot rows (48 x D)."""
     O = np.ascontiguousarray(O, np.float32)
     I = np.ascontiguousarray(I, np.float32)
     mult = _i32arr(mult)
     meta = _i32arr(meta)
-    os_ = O.shape[0]
-    if O.shape[1] != FAST_DIM or I.shape != (FAST_SLOTS, FAST_DIM) or mult.size != FAST_SLOTS or meta.size != os_:
-        raise ValueError("shapes: O (n_out, 128), I (48, 128), mult (48,), meta (n_out,)")
+    os_, D = O.shape
+    if D not in FAST_DIMS or I.shape != (FAST_SLOTS, D) or mult.size != FAST_SLOTS or meta.size != os_:
+        raise ValueError("shapes: O (n_out, D), I (48, D), mult (48,), meta (n_out,), D = 128 or 256")
     corr = np.empty((os_, FAST_SLOTS), np.float32)
-    md = np.empty((os_, FAST_DIM), np.float32)
-    I2 = np.empty((FAST_SLOTS, FAST_DIM), np.float32)
-    _check(lib().dger_sgns_step_fast(os_, _ptr(meta), _ptr(mult), alpha, _ptr(O), _ptr(I), _ptr(corr), _ptr(md),
-                                     _ptr(I2)))
+    md = np.empty((os_, D), np.float32)
+    I2 = np.empty((FAST_SLOTS, D), np.float32)
+    _check(lib().dger_sgns_step_fast_d(D, os_, _ptr(meta), _ptr(mult), alpha, _ptr(O), _ptr(I), _ptr(corr),
+                                       _ptr(md), _ptr(I2)))
     return corr, md, I2
 
 

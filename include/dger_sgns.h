@@ -41,8 +41,9 @@ extern "C" {
 #define DGER_PRECISION_FP32 0   /* FP32 rows, FP32 FMA arithmetic */
 #define DGER_PRECISION_BF16 1   /* BF16 rows in HBM, FP32 working buffers, FP32 FMA contractions */
 #define DGER_PRECISION_BF16_MFMA 2 /* BF16 rows, contractions on MFMA (bf16 operands, FP32 accumulate).
-                                   At size 128, window <= 11, negative <= 14 it runs the window-slot
-                                   trainer: window rows in registers, read from Wih on entering the
+                                   At size 128 or 256, window <= 11, negative <= 14 it runs the
+                                   window-slot trainer (at 256 two wavefronts per worker, one half
+                                   of every row each): window rows in registers, read from Wih on entering the
                                    window and written back (round to nearest, or stochastic
                                    rounding with -rounding sr) on leaving it,
                                    target and negative rows updated in Woh directly, all three
@@ -270,6 +271,14 @@ int dger_sgns_step_mfma_gemm1(int32_t D, int32_t n_out, int32_t n_ctx, int32_t n
  * into one slot. */
 int dger_sgns_step_fast(int32_t n_out, const int32_t *meta, const int32_t *mult, float alpha,
                         const float *O, const float *I, float *corr, float *md, float *I_out);
+
+/* The same step at row length `size` (128 or 256; 128 x becomes size x in
+ * every shape above).  At 256 the step runs as the trainer does at that
+ * size: two wavefronts, each holding one 128-element half of every row, that
+ * add their halves' GEMM1 products through LDS.  Each step of
+ * dsgl.hpp:1241-1333 with repeated contexts merged into one slot. */
+int dger_sgns_step_fast_d(int32_t size, int32_t n_out, const int32_t *meta, const int32_t *mult, float alpha,
+                          const float *O, const float *I, float *corr, float *md, float *I_out);
 
 /* SGNS logistic loss and its adjoint over minibatches of shared negatives,
  * on DEVICE pointers.  Minibatch m pairs contexts ctx[m*n_ctx .. +n_ctx) (rows

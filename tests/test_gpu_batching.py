@@ -2,7 +2,8 @@
 negative sharing (-share 4/8/16: share/2 consecutive steps draw from one
 negative group), minibatches smaller than a step (-batch-parity 0
 -batch-size m), the level-2 autotuner over them, and a caller-supplied HIP
-stream.
+stream; at row lengths 128 and 256 (two wavefronts per worker), the latter
+also with FP32 master rows and stochastic rounding.
 
 The index stream is bit-exact with the oracle's window-slot restatement run
 with the same options (one worker), and the model stays as close to the
@@ -22,33 +23,35 @@ pytestmark = pytest.mark.gpu
 TS = 10_000_000
 
 
-def gpu_opts(W, K, **kw):
-    return G.default_opts(size=128, window=W, negative=K, sample=1e-3, alpha=0.025, iter=1, threads=2,
+def gpu_opts(W, K, D=128, **kw):
+    return G.default_opts(size=D, window=W, negative=K, sample=1e-3, alpha=0.025, iter=1, threads=2,
                           table_size=TS, precision="bf16-mfma", debug=0, **kw)
 
 
 CASES = [
-    # W, K, share, minibatch, corpus kwargs
-    (10, 5, 4, 0, dict(seed=2, lmax=80)),
-    (5, 5, 8, 0, dict(seed=4, zipf=True, V=1200)),
-    (10, 5, 2, 7, dict(seed=2, lmax=80)),
-    (10, 5, 4, 21, dict(seed=6, lmax=70)),
-    (3, 1, 16, 1, dict(seed=9, zipf=True, V=900)),
+    # W, K, share, minibatch, corpus kwargs, row length
+    (10, 5, 4, 0, dict(seed=2, lmax=80), 128),
+    (5, 5, 8, 0, dict(seed=4, zipf=True, V=1200), 128),
+    (10, 5, 2, 7, dict(seed=2, lmax=80), 128),
+    (10, 5, 4, 21, dict(seed=6, lmax=70), 128),
+    (3, 1, 16, 1, dict(seed=9, zipf=True, V=900), 128),
+    (5, 5, 8, 0, dict(seed=4, zipf=True, V=1200), 256),
+    (10, 5, 2, 7, dict(seed=2, lmax=80), 256),
 ]
 
 
-@pytest.mark.parametrize("W,K,share,mb,ck", CASES)
-def test_batching_index_stream_and_model(W, K, share, mb, ck):
+@pytest.mark.parametrize("W,K,share,mb,ck,D", CASES)
+def test_batching_index_stream_and_model(W, K, share, mb, ck, D):
     corpus, cn = random_corpus(**ck)
-    print(f"batching case W={W} K={K} share={share} minibatch={mb} tokens={corpus.size}", flush=True)
-    wih, woh, st, rec, _ = O.train(corpus, cn, O.make_opts(dim=128, window=W, negative=K, sample=1e-3, alpha=0.025,
+    print(f"batching case D={D} W={W} K={K} share={share} minibatch={mb} tokens={corpus.size}", flush=True)
+    wih, woh, st, rec, _ = O.train(corpus, cn, O.make_opts(dim=D, window=W, negative=K, sample=1e-3, alpha=0.025,
                                                           iter=1, threads=2, workers=1, table_size=TS,
                                                           semantics="slots", share=share, minibatch=mb),
                                    rec_cap=60000)
     kw = dict(share=share)
     if mb:
         kw.update(batch_parity=0, batch_size=mb)
-    with G.Trainer(gpu_opts(W, K, **kw), cn, corpus) as tr:
+    with G.Trainer(gpu_opts(W, K, D, **kw), cn, corpus) as tr:
         gst, grec, _ = tr.train(workers=1, rec_cap=60000)
         gwih, gwoh = tr.get_model()
     assert gst.share == share and gst.minibatch == (mb if mb < 4 * W else 0)
@@ -58,6 +61,27 @@ def test_batching_index_stream_and_model(W, K, share, mb, ck):
     assert np.array_equal(grec[0, :n], rec[0, :n])
     # every context is trained once, whatever the minibatch size
     assert gst.contexts == int((rec[0, :n, 4]).sum())
+    for g, r, name in ((gwih, wih, "Wih"), (gwoh, woh, "Woh")):
+        rel = np.linalg.norm(g - r) / np.linalg.norm(r)
+        print(f"  {name}: rel fro vs fp32 window-slot oracle {rel:.4f}")
+        assert rel < 0.1
+
+
+def test_masters_and_stochastic_rounding_at_256():
+    # the full instance with both row halves: FP32 masters for the top 1 %
+    # ranks and stochastic rounding change the arithmetic, not the index
+    # stream, and keep the model near the FP32 trajectory
+    corpus, cn = random_corpus(seed=4, zipf=True, V=1200)
+    print(f"masters + sr at D=256, tokens={corpus.size}", flush=True)
+    wih, woh, st, rec, _ = O.train(corpus, cn, O.make_opts(dim=256, window=5, negative=5, sample=1e-3, alpha=0.025,
+                                                          iter=1, threads=2, workers=1, table_size=TS,
+                                                          semantics="slots"), rec_cap=60000)
+    with G.Trainer(gpu_opts(5, 5, 256, masters=1, rounding=0), cn, corpus) as tr:
+        gst, grec, _ = tr.train(workers=1, rec_cap=60000)
+        gwih, gwoh = tr.get_model()
+    n = int(st.steps)
+    assert gst.steps == st.steps and n > 100
+    assert np.array_equal(grec[0, :n], rec[0, :n])
     for g, r, name in ((gwih, wih, "Wih"), (gwoh, woh, "Woh")):
         rel = np.linalg.norm(g - r) / np.linalg.norm(r)
         print(f"  {name}: rel fro vs fp32 window-slot oracle {rel:.4f}")

@@ -19,8 +19,8 @@ def opts(graph, autotune=0, precision="fp32", D=64):
                           table_size=1_000_000, precision=precision, debug=0, graph=graph, autotune=autotune)
 
 
-# bf16-mfma at size 128 runs the window-slot trainer, at 64 the buffered one
-@pytest.mark.parametrize("precision,D", [("fp32", 64), ("bf16-mfma", 64), ("bf16-mfma", 128)])
+# bf16-mfma at sizes 128 and 256 runs the window-slot trainer, at 64 the buffered one
+@pytest.mark.parametrize("precision,D", [("fp32", 64), ("bf16-mfma", 64), ("bf16-mfma", 128), ("bf16-mfma", 256)])
 def test_graph_replays_match_eager(precision, D):
     corpus, cn = random_corpus(7, V=2000, n_walks=800, lmin=2, lmax=70)
     print(f"graph replay {precision} D={D} tokens={corpus.size}", flush=True)
