@@ -45,8 +45,10 @@ HBM_PEAK_GBS = 8000.0
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
 METRIC = "trained words/sec, BF16-MFMA vs FP32 plain, at \u22640.005 link-pred AUC gap"
-# PMC traffic summary of a trainer kernel on the R-MAT 2^21 workload
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "{kernel}_rmat21_traffic.json")
+# PMC summaries of a trainer kernel on the R-MAT 2^21 workload ({tag}: the
+# kernel name, then "_rmat21", then "_d256" for rows of 256)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "{tag}_traffic.json")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "{tag}_counters.json")
 AUC_GAP_MAX = 0.005
 
 
@@ -162,8 +164,10 @@ def main() -> int:
     row_bytes = {"fp32": 4, "bf16": 2, "bf16-mfma": 2, "fp8": 1}[head]
     bpw = algorithmic_bytes_per_word(D, K, row_bytes, row_extra=4 if head == "fp8" else 0)
     achieved = h["trained_words_per_step"] * bpw / (h["kernel_ms_avg"] * 1e-3) / 1e9
-    # the bf16-mfma form at size 128 runs the window-slot trainer
-    kernel = "k_train_fast" if head == "bf16-mfma" and D == 128 and 2 * W + 1 <= 24 and K + 2 <= 16 else "k_train"
+    # the bf16-mfma form at sizes 128 and 256 runs the window-slot trainer
+    fast = head == "bf16-mfma" and D in (128, 256) and 2 * W + 1 <= 24 and K + 2 <= 16
+    kernel = "k_train_fast" if fast else "k_train"
+    tag = f"{kernel}_rmat21" + ("" if D == 128 else f"_d{D}")
     workload = f"rmat-s{args.scale}-deg16-lognormal3.7-{corpus.n_tokens}tok" + ("" if D == 128 else f"-d{D}")
     peak_tf = BF16_MFMA_PEAK_TFLOPS if head == "bf16-mfma" else FP32_PEAK_TFLOPS
     out = {
@@ -193,7 +197,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(workload, head, kernel, h["trained_words_per_step"]),
+            "traffic": measured_traffic(workload, head, kernel, tag, h["trained_words_per_step"]),
             "bytes_per_word": bpw,
             "S": 2,
             "C_bar": h["C_bar"],
@@ -202,7 +206,7 @@ def main() -> int:
             "tflops": h["contraction_tflops"],
             "peak": peak_tf,
             "frac": h["contraction_tflops"] / peak_tf,
-            "executed_mfma": executed_mfma(workload, head, kernel, h["trained_words_per_step"], h["kernel_ms_avg"]),
+            "executed_mfma": executed_mfma(workload, head, tag, h["trained_words_per_step"], h["kernel_ms_avg"]),
         },
     }
     if "fp32" in results and head != "fp32":
@@ -227,13 +231,13 @@ def main() -> int:
     return 0
 
 
-def measured_traffic(workload, form, kernel, trained_words):
+def measured_traffic(workload, form, kernel, tag, trained_words):
     """HBM bytes per launch from the committed PMC summary (FETCH_SIZE +
     WRITE_SIZE of the trainer, scaled by the copy-kernel calibration), as
     bytes per trained word times this launch's trained words; None if the
     summary does not cover this workload and form."""
     try:
-        with open(TRAFFIC_FILE.format(kernel=kernel)) as f:
+        with open(TRAFFIC_FILE.format(tag=tag)) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
@@ -243,13 +247,13 @@ def measured_traffic(workload, form, kernel, trained_words):
     return e["hbm_bytes_per_word"] * trained_words
 
 
-def executed_mfma(workload, form, kernel, trained_words, kernel_ms):
+def executed_mfma(workload, form, tag, trained_words, kernel_ms):
     """MFMA work the hardware executed (padding included), from the committed
     PMC summary of the kernel: SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 flop per
     step times this launch's steps (two trained words per step), over this
     launch's kernel time; None if the summary does not cover this workload."""
     try:
-        with open(os.path.join(ROOT, "profiles", f"{kernel}_rmat21_counters.json")) as f:
+        with open(COUNTERS_FILE.format(tag=tag)) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
@@ -260,7 +264,7 @@ def executed_mfma(workload, form, kernel, trained_words, kernel_ms):
     tf = flop / (kernel_ms * 1e-3) / 1e12
     return {"tflops": tf, "frac": tf / BF16_MFMA_PEAK_TFLOPS,
             "mfma_busy_cycles_per_step": ps["SQ_VALU_MFMA_BUSY_CYCLES"],
-            "source": f"profiles/{kernel}_rmat21_counters.json"}
+            "source": f"profiles/{tag}_counters.json"}
 
 
 def host_cpus():
