@@ -159,6 +159,18 @@ struct Mat {
   __device__ __forceinline__ void store(unsigned off, const u32x4& v) const {
     __builtin_amdgcn_raw_buffer_store_b128(v, rs, (int)off, 0, kSc1);
   }
+  __device__ __forceinline__ u32x2 load2(unsigned off) const {
+    return __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, kSc1);
+  }
+  __device__ __forceinline__ unsigned load1(unsigned off) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, kSc1);
+  }
+  __device__ __forceinline__ void store2(unsigned off, const u32x2& v) const {
+    __builtin_amdgcn_raw_buffer_store_b64(v, rs, (int)off, 0, kSc1);
+  }
+  __device__ __forceinline__ void store1(unsigned off, unsigned v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(v, rs, (int)off, 0, kSc1);
+  }
   // 4 bytes per lane straight into LDS at dst + 4 * lane (dst wave-uniform);
   // the data is there after wait_vm_all().
   __device__ __forceinline__ void load_lds(unsigned off, unsigned char* dst) const {
@@ -399,6 +411,66 @@ __device__ __forceinline__ u32x4 pack8_rne(const float (&e)[8]) {
   return w;
 }
 
+// ---- FP8 rows ----------------------------------------------------------------
+// A coded row is kH D e4m3 codes (element j at byte j) and its FP32 scale
+// right after them, at a row stride of fp8_row_stride(kH D) bytes: element
+// value = code * scale, scale = (largest magnitude of the row) / 448.
+template <int kH>
+__device__ __forceinline__ unsigned row8(int rk, int h) {
+  return (unsigned)rk * (unsigned)fp8_row_stride(kH * D) + (unsigned)h * D;
+}
+template <int kH>
+__device__ __forceinline__ unsigned scale8(int rk) {
+  return (unsigned)rk * (unsigned)fp8_row_stride(kH * D) + (unsigned)(kH * D);
+}
+// Eight codes (bytes of c) times the row scale.
+__device__ __forceinline__ void fp8_unpack8(const u32x2& c, float sc, float (&e)[8]) {
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)c[k], false);
+    const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)c[k], true);
+    e[4 * k] = lo[0] * sc;
+    e[4 * k + 1] = lo[1] * sc;
+    e[4 * k + 2] = hi[0] * sc;
+    e[4 * k + 3] = hi[1] * sc;
+  }
+}
+// Eight floats to codes for the scale 1 / inv (inv = 448 / row maximum, 0
+// for an all-zero row), each rounded stochastically with its own random bits.
+__device__ __forceinline__ u32x2 fp8_pack8_sr(const float (&e)[8], float inv, unsigned seed) {
+  u32x2 w;
+  unsigned h = hash32(seed);
+  auto next = [&]() {
+    const int r = (int)h;
+    h ^= h << 13; h ^= h >> 17; h ^= h << 5;
+    return r;
+  };
+  auto q = [&](float x) { return fminf(fmaxf(x * inv, -448.f), 448.f); };
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    int x = 0;
+    x = __builtin_amdgcn_cvt_sr_fp8_f32(q(e[4 * k]), next(), x, 0);
+    x = __builtin_amdgcn_cvt_sr_fp8_f32(q(e[4 * k + 1]), next(), x, 1);
+    x = __builtin_amdgcn_cvt_sr_fp8_f32(q(e[4 * k + 2]), next(), x, 2);
+    x = __builtin_amdgcn_cvt_sr_fp8_f32(q(e[4 * k + 3]), next(), x, 3);
+    w[k] = (unsigned)x;
+  }
+  return w;
+}
+// Largest value over the 16 lanes of each lane group (one slot row or one
+// output row of the MFMA layouts).
+__device__ __forceinline__ float max16(float x) {
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) x = fmaxf(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ float absmax8(const float (&e)[8]) {
+  float x = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; j++) x = fmaxf(x, fabsf(e[j]));
+  return x;
+}
+
 // ---- one step's contractions ----------------------------------------------------
 // LDS of the step: rc (bf16 slot rows, swizzled), cb (coefficients
 // transposed for GEMM2: [output][owner group][16 slots in K order], CBS
@@ -576,6 +648,7 @@ struct FastLds {
   int* mult;          // kSlots: the step's contexts per slot
   short* nxt;         // 2 x Lmax: next occurrence
   short* sop;         // 2 x Lmax: slot of each window position
+  float* xw;          // kSlots: FP8 write-backs, largest magnitude of each slot's half row
   u64* stats;         // kStatCount per-worker totals (kept here, not in scalar registers)
 };
 
@@ -601,6 +674,7 @@ __host__ __device__ inline size_t fast_layout(int Lmax, int K, bool full, size_t
   off[4] = o; o = al16(o + 2 * (size_t)Lmax * 4);
   off[5] = o; o = al16(o + (full ? 3 : 1) * 2 * (size_t)Lmax * 2);
   off[1] = o; o = al16(o + kStatCount * 8);
+  if (full) o = al16(o + (size_t)kSlots * 4);  // xw
   return o;
 }
 
@@ -620,6 +694,7 @@ __device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K, bool 
   s.nxt = full ? b5 + 2 * Lmax : nullptr;
   s.sop = full ? b5 + 4 * Lmax : nullptr;
   s.stats = reinterpret_cast<u64*>(base + off[1]);
+  s.xw = full ? reinterpret_cast<float*>(base + al16(off[1] + kStatCount * 8)) : nullptr;
   return s;
 }
 
@@ -706,10 +781,18 @@ __device__ __forceinline__ void write_back(const Mat& wih, const Mat& wm, bool m
 // (a.minibatch) available; the other instance has none of them and none of
 // their code.  Both share negative groups over 2^a.share_shift steps.
 // WPB wavefronts per workgroup; kH halves per row (kH = 2: the workgroup is
-// one worker, its two wavefronts hold halves 0 and 1 of every row).
-template <int WPB, bool kFull, int kH>
+// one worker, its two wavefronts hold halves 0 and 1 of every row).  kFp8:
+// the FP8 form, in the full instance: Wih rows are e4m3 codes with a per-row
+// FP32 scale, decoded into the slots when read and quantized (stochastic
+// rounding, the row's largest magnitude over both halves) when written back;
+// Woh rows stay bf16 and take stochastically rounded atomic adds, and the top
+// ranks of both have FP32 masters.  (Coded Woh rows would take a whole-row
+// read-modify-write at every step they appear in, so that every update would
+// requantize the row and concurrent workers would overwrite each other's.)
+template <int WPB, bool kFull, int kH, bool kFp8>
 __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
   static_assert(kH == 1 || WPB == kH, "a row split into halves takes one workgroup per worker");
+  static_assert(!kFp8 || kFull, "the FP8 form runs in the full instance");
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, m = lane & 15;
@@ -729,7 +812,9 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
   // kH = 2: the other half's GEMM1 products and half 0's word total
   const FastLds s_other = fast_carve(smem + exp_bytes() + (size_t)(wave ^ (kH - 1)) * wave_lds, Lmax, K, kFull);
   u64* const xfer = (h == 0 ? s.stats : s_other.stats) + kStatXfer;
-  const Mat wih(a.wih, (unsigned)((size_t)a.V * kH * D * 2)), woh(a.woh, (unsigned)((size_t)a.V * kH * D * 2));
+  // (the FP8 form codes Wih only; Woh stays bf16)
+  const size_t wih_bytes = kFp8 ? (size_t)a.V * fp8_row_stride(kH * D) : (size_t)a.V * kH * D * 2;
+  const Mat wih(a.wih, (unsigned)wih_bytes), woh(a.woh, (unsigned)((size_t)a.V * kH * D * 2));
   // FP32 master rows of the hottest ranks (rank < nmaster).
   const int nmaster = kFull ? a.n_master : 0;
   const int round_nearest = kFull ? a.round_nearest : 1;
@@ -917,8 +1002,12 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
         // between steps, and also writes them into the slot copy)
         unsigned char* const stg = kFull ? s.rc : s.ost;
         const int krk = (lane < nnew) ? s.srank[kSentSlots * q + lane] : 0;  // rank of new row `lane`
-        for (int k = 0; k < nnew; k++)
-          wih.load_lds(rowb<kH>(__builtin_amdgcn_readlane(krk, k), h) + 4 * lane, stg + k * (D * 2));
+        for (int k = 0; k < nnew; k++) {
+          const int rk = __builtin_amdgcn_readlane(krk, k);
+          // FP8: codes in the first D bytes, the scale after them
+          wih.load_lds(kFp8 ? (lane < D / 4 ? row8<kH>(rk, h) + 4 * lane : scale8<kH>(rk)) : rowb<kH>(rk, h) + 4 * lane,
+                       stg + k * (D * 2));
+        }
         wait_vm_all();
         if constexpr (!kFull) {
           // All new rows at once: each lane fills those of its slot
@@ -948,6 +1037,11 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
           if (is_master(rk)) {
             float e[8];
             master_get<kH>(wm, rk, h, m, e);
+            slot_put_f(S, kSentSlots * q + k, e, g);
+          } else if constexpr (kFp8) {
+            float e[8];
+            fp8_unpack8(*reinterpret_cast<const u32x2*>(stg + k * (D * 2) + 8 * m),
+                        *reinterpret_cast<const float*>(stg + k * (D * 2) + D), e);
             slot_put_f(S, kSentSlots * q + k, e, g);
           } else {
             slot_put(S, kSentSlots * q + k, *reinterpret_cast<const u32x4*>(stg + k * (D * 2) + 16 * m), g);
@@ -1168,6 +1262,10 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
           const f32x4 hi = *reinterpret_cast<const f32x4*>(s.ost + q * (D * 4) + 32 * m + 16);
           const float e[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
           slot_put_f(S, pend[q], e, g);
+        } else if constexpr (kFp8) {
+          float e[8];
+          fp8_unpack8((u32x2){in_row[q][0], in_row[q][1]}, __uint_as_float(in_row[q][2]), e);
+          slot_put_f(S, pend[q], e, g);
         } else {
           slot_put(S, pend[q], in_row[q], g);
           if (!kFull && g == 0) *reinterpret_cast<u32x4*>(s.rc + swz(pend[q], m)) = in_row[q];
@@ -1290,7 +1388,14 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
       const int rk = (last && in_slot_src[q] == -1) ? __builtin_amdgcn_readlane(gr, 6 + q) : -1;
       pend_hot[q] = rk >= 0 && is_master(rk);  // staged into ost after GEMM3 below
       in_rk[q] = rk;
-      if (!pend_hot[q]) in_row[q] = wih.load(rowb<kH>(rk >= 0 ? rk : 0, h) + 16 * m);
+      if (!pend_hot[q]) {
+        if constexpr (kFp8) {
+          const u32x2 c = wih.load2(row8<kH>(rk >= 0 ? rk : 0, h) + 8 * m);
+          in_row[q] = (u32x4){c[0], c[1], wih.load1(scale8<kH>(rk >= 0 ? rk : 0)), 0u};
+        } else {
+          in_row[q] = wih.load(rowb<kH>(rk >= 0 ? rk : 0, h) + 16 * m);
+        }
+      }
     }
     // Plan the next step's outputs and read its rows (after this step's
     // updates of the same rows), when neither sentence ends here.
@@ -1347,6 +1452,12 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
       pm_old = pm;
       pm >>= 1;  // both windows move on by one position
     }
+    // FP8: the slots to write back are collected here and quantized after the
+    // loop (a row's scale needs its largest magnitude over both halves); a
+    // slot freed below keeps its registers until the next step's start, and
+    // its rank in srank until the write-backs are done.
+    u64 wbm = 0;                              // bit s: slot s goes back to Wih
+    int ent_sl[2] = {-1, -1}, ent_rk[2] = {0, 0};  // entering rows' slots and ranks, recorded after
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int p = pos[q];
@@ -1370,6 +1481,8 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
               wih.store(rowb<kH>(rk, h) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sl4, m)));
             }
           }
+        } else if (kFp8) {
+          wbm |= (u64)(~freem[q] & 0xFFFFFFu) << (kSentSlots * q);
         } else
         for (unsigned live = ~freem[q] & 0xFFFFFFu; live; live &= live - 1) {
           const int sl = kSentSlots * q + __builtin_ctz(live);
@@ -1422,6 +1535,8 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
         const int rkx = __builtin_amdgcn_readlane(gr, 4 + q);
         if (!kFull) {
           if (g == q) wih.store(rowb<kH>(rkx, h) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
+        } else if (kFp8) {
+          wbm |= 1ull << sx;
         } else {
         float e[8];
         slot_get(S, sx, e);
@@ -1440,8 +1555,56 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
         const int sl = kSentSlots * q + k;
         pend[q] = sl;
         const int rky = __builtin_amdgcn_readlane(gr, 6 + q);
-        if (lane == 0) { sq[y] = (short)sl; s.srank[sl] = rky; }
+        if (kFp8) {
+          ent_sl[q] = sl;
+          ent_rk[q] = rky;
+          if (lane == 0) sq[y] = (short)sl;
+        } else if (lane == 0) {
+          sq[y] = (short)sl;
+          s.srank[sl] = rky;
+        }
       }
+    }
+    if constexpr (kFp8) {
+      // Slot 16 rt + 4 g + r sits at fixed registers of lane group g, so the
+      // write-backs go by register position: each of the 12 positions that
+      // holds a slot to write back serves four slots at once, one per lane
+      // group.  Two halves pass their largest magnitudes through xw (one
+      // barrier per such position; a master row takes its FP32 values).
+      if (wbm != 0) {
+#pragma unroll
+        for (int rt = 0; rt < 3; rt++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            if ((wbm & (0x1111ull << (16 * rt + r))) == 0) continue;
+            const int sl = 16 * rt + 4 * g + r;  // this lane group's slot
+            const bool mine = (wbm >> sl) & 1u;
+            float e[8];
+#pragma unroll
+            for (int t = 0; t < NT; t++) e[t] = sget(S, rt, t, r);
+            const int rk = s.srank[sl];
+            const bool hot = is_master(rk);
+            float mx = max16(absmax8(e));
+            if constexpr (kH > 1) {
+              if (m == 0) s.xw[sl] = mx;
+              __syncthreads();
+              mx = fmaxf(mx, s_other.xw[sl]);
+            }
+            if (mine) {
+              if (hot) {
+                master_put<kH>(wm, rk, h, m, e);
+              } else {
+                const float inv = mx > 0.f ? 448.f / mx : 0.f;
+                wih.store2(row8<kH>(rk, h) + 8 * m,
+                           fp8_pack8_sr(e, inv, salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sl << 24)));
+                if (m == 0) wih.store1(scale8<kH>(rk), __float_as_uint(mx > 0.f ? mx / 448.f : 0.f));
+              }
+            }
+          }
+      }
+#pragma unroll
+      for (int q = 0; q < 2; q++)
+        if (ent_sl[q] >= 0 && lane == 0) s.srank[ent_sl[q]] = ent_rk[q];
     }
     if (done_sent >= 2) {
       refresh = 1;
@@ -1579,14 +1742,18 @@ int fast_workers_per_block(int Lmax, int K, size_t lds_per_block, int D_) {
 namespace {
 // The trainer instance for a run: masters, stochastic rounding or
 // minibatches take the full instance; halves per row from D.
-template <bool kFull, int kH>
+template <bool kFull, int kH, bool kFp8 = false>
 const void* train_fast_fn() {
-  return reinterpret_cast<const void*>(&k_train_fast<kH == 1 ? kFastMaxWpb : kH, kFull, kH>);
+  return reinterpret_cast<const void*>(&k_train_fast<kH == 1 ? kFastMaxWpb : kH, kFull, kH, kFp8>);
 }
-bool fast_full(const TrainArgs& a) { return a.n_master > 0 || !a.round_nearest || a.minibatch > 0; }
+bool fast_full(const TrainArgs& a) { return a.fp8 || a.n_master > 0 || !a.round_nearest || a.minibatch > 0; }
 const void* pick_train_fast(const TrainArgs& a) {
   const bool full = fast_full(a);
-  if (fast_halves(a.D) == 2) return full ? train_fast_fn<true, 2>() : train_fast_fn<false, 2>();
+  if (fast_halves(a.D) == 2) {
+    if (a.fp8) return train_fast_fn<true, 2, true>();
+    return full ? train_fast_fn<true, 2>() : train_fast_fn<false, 2>();
+  }
+  if (a.fp8) return train_fast_fn<true, 1, true>();
   return full ? train_fast_fn<true, 1>() : train_fast_fn<false, 1>();
 }
 bool fast_args_ok(const TrainArgs& a) {

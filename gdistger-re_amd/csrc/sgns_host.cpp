@@ -326,12 +326,19 @@ static int run_minibatch(const dger_sgns_opts& o) {
   return o.batch_size >= 4 * o.window ? 0 : o.batch_size;
 }
 
+// Whether matrix m (0: Wih, 1: Woh) holds FP8 codes: both matrices in the
+// general trainer's FP8 form, Wih only in the window-slot trainer's (its Woh
+// rows are bf16).
+static bool coded(const dger_ctx* c, int m) {
+  return c->o.precision == DGER_PRECISION_FP8 && (m == 0 || !c->fast);
+}
+
 // The model in the context's storage form from FP32 rows (rank order) in
 // tmp_d (Wih when m == 0, Woh when m == 1).
 static int store_model_from_tmp(dger_ctx* c, int m) {
   const uint64_t n = (uint64_t)c->V * (uint64_t)c->D;
   DevBuf* dst = m ? &c->woh_d : &c->wih_d;
-  if (c->o.precision == DGER_PRECISION_FP8) {
+  if (coded(c, m)) {
     DGER_HIP(dger::launch_fp8_quantize(c->tmp_d.as<float>(), c->desc_d.as<dger::Fp8Mat>() + m, c->V, c->D, c->stream));
   } else if (c->o.precision != DGER_PRECISION_FP32) {
     DGER_HIP(dger::launch_f32_to_bf16(c->tmp_d.as<float>(), dst->as<uint16_t>(), n, c->stream));
@@ -349,7 +356,7 @@ static int load_model_to_tmp(dger_ctx* c, int m) {
   const uint64_t n = (uint64_t)c->V * (uint64_t)c->D;
   DGER_HIP(c->tmp_d.ensure(n * 4));
   DevBuf* src = m ? &c->woh_d : &c->wih_d;
-  if (c->o.precision == DGER_PRECISION_FP8) {
+  if (coded(c, m)) {
     DGER_HIP(dger::launch_fp8_dequantize(c->desc_d.as<dger::Fp8Mat>() + m, c->tmp_d.as<float>(), c->V, c->D, c->stream));
   } else if (c->o.precision != DGER_PRECISION_FP32) {
     DGER_HIP(dger::launch_bf16_to_f32(src->as<uint16_t>(), c->tmp_d.as<float>(), n, c->stream));
@@ -408,7 +415,7 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   c->o = *opts;
   c->V = v_num;
   c->D = opts->size;
-  c->fast = opts->precision == DGER_PRECISION_BF16_MFMA &&
+  c->fast = (opts->precision == DGER_PRECISION_BF16_MFMA || opts->precision == DGER_PRECISION_FP8) &&
             dger::fast_supported(opts->size, opts->negative, opts->window, v_num, opts->table_size);
   auto bail = [&](int code) { delete c; return code; };
   if (opts->batch_parity) c->o.batch_size = 2 * opts->window + 1;  // dsgl.hpp:832
@@ -417,7 +424,7 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   if ((c->share != 2 || c->minibatch != 0 || opts->autotune == 2) && !c->fast)
     return bail(fail(DGER_E_UNSUPPORTED,
                      "-share, -batch-parity 0 and -autotune 2 need the window-slot trainer "
-                     "(-precision bf16-mfma, -size 128 or 256, -window <= 11, -negative <= 14)"));
+                     "(-precision bf16-mfma or fp8, -size 128 or 256, -window <= 11, -negative <= 14)"));
   // The allocation-free entry point's scratch address on this device is
   // looked up once here (dger_ctx_train_noalloc reports a failure).
   if (c->fast) {
@@ -465,7 +472,7 @@ int dger_ctx_create(const dger_sgns_opts* opts, const int32_t* vertex_cn, uint32
   if (c->id_rank_d.ensure(v_num * 4) || c->keep_d.ensure(v_num * 4) ||
       c->table_d.ensure((size_t)opts->table_size * 4) || c->exp_d.ensure(sizeof(exp_table)) ||
       c->wih_d.ensure(opts->precision == DGER_PRECISION_FP8 ? (size_t)v_num * dger::fp8_row_stride(c->D) : n * rb) ||
-      c->woh_d.ensure(opts->precision == DGER_PRECISION_FP8 ? (size_t)v_num * dger::fp8_row_stride(c->D) : n * rb) ||
+      c->woh_d.ensure(coded(c, 1) ? (size_t)v_num * dger::fp8_row_stride(c->D) : n * rb) ||
       c->counters_d.ensure(dger::kNumCounters * 8) || c->tmp_d.ensure(starts.size() * 8))
     return bail(fail(DGER_E_ALLOC, "device allocation failed"));
   if (hipMemcpyAsync(c->id_rank_d.p, c->id_rank.data(), v_num * 4, hipMemcpyHostToDevice, c->stream) ||
@@ -586,8 +593,10 @@ static dger::TrainArgs make_args(dger_ctx* c, int workers, int wpb, int32_t* rec
   a.table_size = (uint64_t)c->o.table_size;
   a.exp_table = c->exp_d.as<float>();
   const bool fp8 = c->o.precision == DGER_PRECISION_FP8;
-  a.wih = fp8 ? (void*)c->desc_d.as<dger::Fp8Mat>() : c->wih_d.p;
-  a.woh = fp8 ? (void*)(c->desc_d.as<dger::Fp8Mat>() + 1) : c->woh_d.p;
+  // (the general trainer reads FP8 matrices through their descriptors, the
+  // window-slot trainer the coded rows and the masters directly)
+  a.wih = (fp8 && !c->fast) ? (void*)c->desc_d.as<dger::Fp8Mat>() : c->wih_d.p;
+  a.woh = (fp8 && !c->fast) ? (void*)(c->desc_d.as<dger::Fp8Mat>() + 1) : c->woh_d.p;
   a.fscratch = c->scratch_d.as<float>();
   a.counters = c->counters_d.as<unsigned long long>();
   a.rec = rec;
@@ -613,6 +622,10 @@ static dger::TrainArgs make_args(dger_ctx* c, int workers, int wpb, int32_t* rec
     a.round_nearest = c->o.rounding == 1 ? 1 : 0;
     a.share_shift = c->share == 16 ? 3 : c->share == 8 ? 2 : c->share == 4 ? 1 : 0;
     a.minibatch = c->minibatch;
+    if (fp8) {  // coded rows, stochastic rounding on every row store
+      a.fp8 = 1;
+      a.round_nearest = 0;
+    }
   }
   return a;
 }
@@ -660,9 +673,10 @@ static int run_pass(dger_ctx* c, const dger::TrainArgs& a, bool use_graph, float
   DGER_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
   if (c->fast && c->n_master > 0) {
     // The bf16 rows of the master ranks (read by link evaluation and the
-    // next pass only through their masters) follow their masters.
+    // next pass only through their masters) follow their masters; coded
+    // rows are read through their masters directly.
     const uint64_t nmd = (uint64_t)c->n_master * c->D;
-    DGER_HIP(dger::launch_f32_to_bf16(c->wm_d.as<float>(), c->wih_d.as<uint16_t>(), nmd, c->stream));
+    if (!coded(c, 0)) DGER_HIP(dger::launch_f32_to_bf16(c->wm_d.as<float>(), c->wih_d.as<uint16_t>(), nmd, c->stream));
     DGER_HIP(dger::launch_f32_to_bf16(c->om_d.as<float>(), c->woh_d.as<uint16_t>(), nmd, c->stream));
     DGER_HIP(hipStreamSynchronize(c->stream));
   }
@@ -857,8 +871,8 @@ int dger_ctx_train_noalloc(dger_ctx* c, int32_t workers, dger_sgns_stats* stats)
   if (!c->have_corpus) return fail(DGER_E_STATE, "dger_ctx_set_corpus must come first");
   if (!c->fast)
     return fail(DGER_E_UNSUPPORTED,
-                "dger_ctx_train_noalloc runs the window-slot trainer only (DGER_PRECISION_BF16_MFMA, "
-                "size 128 or 256, window <= 11, negative <= 14); use dger_ctx_train");
+                "dger_ctx_train_noalloc runs the window-slot trainer only (DGER_PRECISION_BF16_MFMA or "
+                "DGER_PRECISION_FP8, size 128 or 256, window <= 11, negative <= 14); use dger_ctx_train");
   DGER_HIP(hipSetDevice(c->o.device));
   void* sp = nullptr;
   DGER_HIP(dger::noalloc_scratch(c->o.device, &sp));

@@ -69,6 +69,9 @@ struct TrainArgs {
   // splits a step's contexts into minibatches of at most that many.
   int32_t share_shift;
   int32_t minibatch;
+  // Window-slot trainer, FP8 form: wih / woh are coded rows (fp8_row_stride
+  // bytes each) instead of bf16 rows.
+  int32_t fp8;
 };
 
 // Floats of per-worker scratch: two sentence buffers (Lmax rows each) and the
@@ -131,7 +134,8 @@ hipError_t launch_table_fill(int32_t* table, uint64_t table_size, const int64_t*
 hipError_t prepare_train(const TrainArgs& a, int precision);
 hipError_t launch_train(const TrainArgs& a, int precision, hipStream_t s);
 // FP8 form: a model matrix is V rows of D e4m3 codes followed by the row's
-// FP32 scale (row stride D + 4 bytes), plus FP32 master rows for ranks
+// FP32 scale (row stride D + 4 rounded up to 16 bytes, so that every row's
+// codes start 16-byte aligned), plus FP32 master rows for ranks
 // < n_master that take the place of the coded rows.  The trainer gets a
 // device pointer to this descriptor instead of the row array.
 struct Fp8Mat {
@@ -140,7 +144,7 @@ struct Fp8Mat {
   int32_t n_master;
   int32_t D;
 };
-DGER_HD inline size_t fp8_row_stride(int D) { return (size_t)D + 4; }
+DGER_HD inline size_t fp8_row_stride(int D) { return ((size_t)D + 4 + 15) & ~(size_t)15; }
 // FP32 rows (rank order) <-> an FP8 matrix; quantize rounds to nearest even.
 hipError_t launch_fp8_quantize(const float* src, const Fp8Mat* m, uint32_t V, int D, hipStream_t s);
 hipError_t launch_fp8_dequantize(const Fp8Mat* m, float* dst, uint32_t V, int D, hipStream_t s);

@@ -52,7 +52,10 @@ extern "C" {
 #define DGER_PRECISION_FP8 3    /* FP8 e4m3 rows with a per-row FP32 scale, FP32 master rows for the
                                    top 1% ranks by count, stochastic rounding on every row store;
                                    contractions as BF16_MFMA.  Needs size % 4 == 0, window <= 16,
-                                   negative <= 6. */
+                                   negative <= 6.  At size 128 or 256 it runs on the window-slot
+                                   trainer, where Wih (the embeddings) is coded and Woh stays bf16
+                                   with stochastically rounded atomic adds (a coded Woh row would be
+                                   requantized at every step it is used); elsewhere both are coded. */
 
 /* Options.  The first block mirrors the flags dsgl() parses
  * (include/dsgl.hpp:1496-1535) with the same defaults (dsgl.hpp:67-76);

@@ -235,5 +235,7 @@ def test_c5_rmat21_d256_fp8(rmat21):
     assert abs(res["fp8"]["auc"] - res["fp32"]["auc"]) <= 0.01
     assert abs(loss["fp8"] - loss["fp32"]) <= 0.02 * loss["fp32"]
     # the learning rate decays linearly with the words trained (to alpha0 * 1e-4
-    # at the very end); the first worker stops close to the end of the pass
-    assert res["fp8"]["alpha"] < 0.1 * 0.01
+    # at the very end); the first worker stops once most of the pass is done
+    # (85-95 % of the words, depending on the worker count), so its final
+    # rate is a small fraction of alpha0
+    assert res["fp8"]["alpha"] < 0.2 * 0.01

@@ -5,15 +5,25 @@ ranks, stochastic rounding on every row store).
   bound (half a step of the 3-bit mantissa, or the subnormal step below
   2^-6 of the row scale);
 * one worker: the index stream equals the FP32 form's (indices never depend
-  on model values) and the model stays near the FP32 trajectory;
-* link prediction on an R-MAT corpus at D = 256 (the precision-stress shape):
-  AUC within 0.01 of the FP32 form."""
+  on model values) and the model stays near the FP32 trajectory: at size 64
+  the general trainer's FP32 run (the reference's update rules), at 128 and
+  256 (the window-slot trainer, which takes the FP8 form at those sizes) the
+  oracle's FP32 run of the window-slot rules;
+* link prediction on an R-MAT 2^15 corpus (3M tokens, a worker count set
+  by the corpus, ~750): at D = 64 (general trainer) AUC within 0.01 of the
+  FP32 form; at D = 256 (the precision-stress shape, window-slot trainer)
+  within 0.01 of the BF16-MFMA form on the same trainer, so that the check
+  isolates the row format.  (On a corpus this small, many Hogwild workers
+  lower the window-slot trainer's AUC below the reference-rule trainers'
+  whatever the row format; the FP8 form is checked against FP32 at the full
+  C5 size in test_gpu_configs.py.)"""
 import numpy as np
 import pytest
 
 import dger_sgns as G
 from corpora import random_corpus
 from dger_sgns import synth
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -23,10 +33,11 @@ def opts(D, W, K, precision, ts=10_000_000, alpha=0.025):
                           table_size=ts, precision=precision, debug=0)
 
 
-def test_fp8_storage_round_trip():
+@pytest.mark.parametrize("D", [64, 256])
+def test_fp8_storage_round_trip(D):
     corpus, cn = random_corpus(3, V=1500, n_walks=300)
     rng = np.random.default_rng(4)
-    V, D = cn.size, 64
+    V = cn.size
     x = (rng.normal(0, 1, (V, D)) * rng.lognormal(0, 2, (V, 1))).astype(np.float32)
     x[5] = 0.0  # an all-zero row keeps scale 0
     with G.Trainer(opts(D, 5, 5, "fp8"), cn, corpus) as tr:
@@ -42,30 +53,41 @@ def test_fp8_storage_round_trip():
     assert np.array_equal(woh[:n_master], x[::-1][:n_master])
 
 
-def test_fp8_same_index_stream_close_model():
+@pytest.mark.parametrize("D", [64, 128, 256])
+def test_fp8_same_index_stream_close_model(D):
     corpus, cn = random_corpus(9, n_walks=800)
-    with G.Trainer(opts(128, 5, 5, "fp32"), cn, corpus) as tr:
-        st, rec, _ = tr.train(workers=1, rec_cap=30000)
-        wih, woh = tr.get_model()
-    with G.Trainer(opts(128, 5, 5, "fp8"), cn, corpus) as tr:
+    if D in G.FAST_DIMS:
+        # the window-slot trainer's rules, in FP32 on the CPU
+        wih, woh, st, rec, _ = O.train(corpus, cn, O.make_opts(dim=D, window=5, negative=5, sample=1e-3, alpha=0.025,
+                                                                iter=1, threads=2, workers=1, table_size=10_000_000,
+                                                                semantics="slots"), rec_cap=30000)
+        rec = rec[0]
+    else:
+        with G.Trainer(opts(D, 5, 5, "fp32"), cn, corpus) as tr:
+            st, rec, _ = tr.train(workers=1, rec_cap=30000)
+            wih, woh = tr.get_model()
+        rec = rec[0]
+    with G.Trainer(opts(D, 5, 5, "fp8"), cn, corpus) as tr:
         gst, grec, _ = tr.train(workers=1, rec_cap=30000)
         gwih, gwoh = tr.get_model()
     n = int(st.steps)
-    assert gst.steps == st.steps and np.array_equal(grec[0, :n], rec[0, :n])
-    for g, r, bound in ((gwih, wih, 0.25), (gwoh, woh, 0.40)):
+    assert n > 100
+    assert gst.steps == st.steps and np.array_equal(grec[0, :n], rec[:n])
+    for g, r, bound, name in ((gwih, wih, 0.25, "Wih"), (gwoh, woh, 0.40, "Woh")):
         rel = np.linalg.norm(g - r) / np.linalg.norm(r)
-        print(f"fp8 vs fp32: rel fro {rel:.4f}", flush=True)
+        print(f"fp8 D={D} {name} vs fp32: rel fro {rel:.4f}", flush=True)
         assert np.isfinite(g).all() and rel <= bound
 
 
-def test_fp8_link_prediction_close_to_fp32():
+@pytest.mark.parametrize("D,ref", [(64, "fp32"), (256, "bf16-mfma")])
+def test_fp8_link_prediction_close_to_wider_rows(D, ref):
     c, pairs, labels = synth.rmat_corpus(scale=15, seed=5, target_tokens=3_000_000, holdout=0.01)
-    print(f"fp8 AUC corpus tokens={c.n_tokens} pairs={pairs.shape[0]}", flush=True)
+    print(f"fp8 AUC corpus tokens={c.n_tokens} pairs={pairs.shape[0]} D={D}", flush=True)
     auc = {}
-    for precision in ("fp32", "fp8"):
-        with G.Trainer(opts(256, 10, 5, precision, ts=100_000_000, alpha=0.01), c.vertex_cn, c.tokens) as tr:
+    for precision in (ref, "fp8"):
+        with G.Trainer(opts(D, 10, 5, precision, ts=100_000_000, alpha=0.01), c.vertex_cn, c.tokens) as tr:
             tr.train()
             auc[precision] = tr.link_auc(pairs, labels)
-    print(f"AUC fp32 {auc['fp32']:.4f} fp8 {auc['fp8']:.4f}", flush=True)
-    assert auc["fp32"] > 0.7
-    assert abs(auc["fp8"] - auc["fp32"]) <= 0.01
+    print(f"AUC {ref} {auc[ref]:.4f} fp8 {auc['fp8']:.4f}", flush=True)
+    assert auc[ref] > 0.7
+    assert abs(auc["fp8"] - auc[ref]) <= 0.01
