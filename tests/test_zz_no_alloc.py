@@ -9,7 +9,8 @@ dger_ctx_train on the same contexts and corpora.
 * Two calls in a row on one GPU (the static scratch keeps the last call's
   contents), the second on a smaller corpus.
 * Several workers: the statistics are equal (each worker's index stream
-  depends only on its slice).
+  depends only on its slice), also at row length 256 (two wavefronts per
+  worker, twice the scratch) and in the FP8 form.
 * Host-side rejections: a context the window-slot trainer does not run, and a
   worker count whose scratch exceeds the static array."""
 import numpy as np
@@ -24,13 +25,13 @@ TS = 10_000_000
 STAT_FIELDS = ("steps", "trained_words", "raw_words", "sentences", "contexts", "out_ctx", "workers", "final_alpha")
 
 
-def opts(precision="bf16-mfma"):
-    return G.default_opts(size=128, window=5, negative=5, sample=1e-3, alpha=0.025, iter=1, threads=2,
+def opts(precision="bf16-mfma", D=128):
+    return G.default_opts(size=D, window=5, negative=5, sample=1e-3, alpha=0.025, iter=1, threads=2,
                           table_size=TS, precision=precision, debug=0)
 
 
-def trained(corpus, cn, workers, noalloc):
-    with G.Trainer(opts(), cn, corpus) as tr:
+def trained(corpus, cn, workers, noalloc, **kw):
+    with G.Trainer(opts(**kw), cn, corpus) as tr:
         st = tr.train_noalloc(workers=workers) if noalloc else tr.train(workers=workers)[0]
         wih, woh = tr.get_model()
     return st, wih, woh
@@ -62,11 +63,12 @@ def test_noalloc_one_worker_two_calls():
                 assert np.abs(x - y).max() <= 4 * run_to_run + 1e-6
 
 
-def test_noalloc_several_workers_stats():
+@pytest.mark.parametrize("precision,D", [("bf16-mfma", 128), ("bf16-mfma", 256), ("fp8", 256)])
+def test_noalloc_several_workers_stats(precision, D):
     corpus, cn = random_corpus(12, n_walks=2000, zipf=True)
-    print(f"noalloc vs train, 16 workers, {corpus.size} tokens", flush=True)
-    sa, wa, _ = trained(corpus, cn, 16, noalloc=False)
-    sb, wb, _ = trained(corpus, cn, 16, noalloc=True)
+    print(f"noalloc vs train, 16 workers, {precision} D={D}, {corpus.size} tokens", flush=True)
+    sa, wa, _ = trained(corpus, cn, 16, noalloc=False, precision=precision, D=D)
+    sb, wb, _ = trained(corpus, cn, 16, noalloc=True, precision=precision, D=D)
     check_stats(sa, sb)
     assert np.isfinite(wb).all()
     rel = np.linalg.norm(wa - wb) / np.linalg.norm(wa)
