@@ -198,11 +198,18 @@ __device__ inline int occurrences_near(const int* v, short* prev, int n, int nfi
     const int p = c0 + lane;
     const int x = (p < n) ? v[p] : -1;
     int pv = -1;
-#pragma unroll 4
-    for (int d = 1; d <= reach; d++) {
-      const int q = p - d;
-      const int y = v[min(max(q, 0), n - 1)];
-      pv = (pv < 0 && q >= 0 && y == x) ? q : pv;
+    // farthest first, four reads at a time, so that the nearest match is the
+    // last one kept and the reads of a group are all in flight (no read
+    // depends on a comparison)
+    for (int d0 = reach; d0 >= 1; d0 -= 4) {
+      int y[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) y[j] = v[min(max(p - (d0 - j), 0), n - 1)];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int d = d0 - j, q = p - d;
+        pv = ((y[j] == x) & (q >= 0) & (d >= 1)) ? q : pv;
+      }
     }
     if (p < n) prev[p] = (short)pv;
   }
@@ -517,15 +524,22 @@ __device__ __forceinline__ void gemm1_coef(const unsigned char* ost, const StepL
   f32x4 f[3];
 #pragma unroll
   for (int rt = 0; rt < 3; rt++) f[rt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  // Every operand is read from LDS before the first MFMA (left to itself
+  // the scheduler keeps only one or two reads ahead of the MFMAs, so that
+  // each waits out an LDS round trip).
+  bf16x8 av[KB1], bv[KB1][3];
 #pragma unroll
   for (int ks = 0; ks < KB1; ks++) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(ost + swz(m, 4 * ks + g));
+    av[ks] = *reinterpret_cast<const bf16x8*>(ost + swz(m, 4 * ks + g));
 #pragma unroll
-    for (int rt = 0; rt < 3; rt++) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(L.rc + swz(16 * rt + m, 4 * ks + g));
-      f[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, f[rt], 0, 0, 0);
-    }
+    for (int rt = 0; rt < 3; rt++) bv[ks][rt] = *reinterpret_cast<const bf16x8*>(L.rc + swz(16 * rt + m, 4 * ks + g));
   }
+  __builtin_amdgcn_sched_group_barrier(0x100, 4 * KB1, 0);  // the 16 LDS reads
+#pragma unroll
+  for (int ks = 0; ks < KB1; ks++)
+#pragma unroll
+    for (int rt = 0; rt < 3; rt++) f[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[ks], bv[ks][rt], f[rt], 0, 0, 0);
+  __builtin_amdgcn_sched_group_barrier(0x008, 3 * KB1, 0);  // then the 12 MFMAs
   if constexpr (kH > 1) {
     const int lane = 16 * g + m;
 #pragma unroll
