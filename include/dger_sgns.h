@@ -206,11 +206,12 @@ int dger_ctx_train(dger_ctx *ctx, int32_t workers, int32_t *rec, int64_t rec_cap
 /* dger_ctx_train without any device allocation (no counterpart in the
  * reference): for callers that cannot allocate, such as code under HIP graph
  * capture, tight serving loops or harnesses that forbid hipMalloc.  Runs the
- * window-slot trainer (DGER_PRECISION_BF16_MFMA contexts it supports; other
- * contexts: DGER_E_UNSUPPORTED) with its per-run scratch -- counters, per-worker
- * step counts and rates, per-worker negative batches -- in a static device
- * array of 16 MiB per GPU (held until the process ends), so that
- * 4 * (negative * max_sentence + 4) bytes per worker plus 2 KiB must fit:
+ * window-slot trainer (DGER_PRECISION_BF16_MFMA and DGER_PRECISION_FP8
+ * contexts it supports; other contexts: DGER_E_UNSUPPORTED) with its per-run
+ * scratch -- counters, per-worker step counts and rates, per-wavefront
+ * negative batches -- in a static device array of 16 MiB per GPU (held until
+ * the process ends), so that 4 * (negative * max_sentence * size / 128 + 3)
+ * bytes per worker plus 2 KiB must fit:
  * larger requests fail with DGER_E_ARG and name the worker count that fits.
  * No autotuning, records or graph replay; stats as dger_ctx_train.  Every call
  * on one GPU uses that GPU's array: calls that overlap on one GPU race. */
