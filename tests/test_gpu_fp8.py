@@ -91,3 +91,27 @@ def test_fp8_link_prediction_close_to_wider_rows(D, ref):
     print(f"AUC {ref} {auc[ref]:.4f} fp8 {auc['fp8']:.4f}", flush=True)
     assert auc[ref] > 0.7
     assert abs(auc["fp8"] - auc[ref]) <= 0.01
+
+
+def test_fp8_with_sharing_and_minibatches_at_256():
+    # the FP8 form runs in the full instance, which also takes negative
+    # sharing and minibatches: the index stream stays the oracle's for the
+    # same options, and the model near its FP32 run of the window-slot rules
+    corpus, cn = random_corpus(2, lmax=80)
+    kw = dict(share=4, batch_parity=0, batch_size=7)
+    print(f"fp8 D=256 share 4 minibatch 7, tokens={corpus.size}", flush=True)
+    wih, woh, st, rec, _ = O.train(corpus, cn, O.make_opts(dim=256, window=10, negative=5, sample=1e-3, alpha=0.025,
+                                                            iter=1, threads=2, workers=1, table_size=10_000_000,
+                                                            semantics="slots", share=4, minibatch=7), rec_cap=60000)
+    o = G.default_opts(size=256, window=10, negative=5, sample=1e-3, alpha=0.025, iter=1, threads=2,
+                       table_size=10_000_000, precision="fp8", debug=0, **kw)
+    with G.Trainer(o, cn, corpus) as tr:
+        gst, grec, _ = tr.train(workers=1, rec_cap=60000)
+        gwih, gwoh = tr.get_model()
+    n = int(st.steps)
+    assert gst.share == 4 and gst.minibatch == 7
+    assert gst.steps == st.steps and n > 100 and np.array_equal(grec[0, :n], rec[0, :n])
+    for g, r, bound, name in ((gwih, wih, 0.25, "Wih"), (gwoh, woh, 0.40, "Woh")):
+        rel = np.linalg.norm(g - r) / np.linalg.norm(r)
+        print(f"  {name}: rel fro vs fp32 window-slot oracle {rel:.4f}", flush=True)
+        assert np.isfinite(g).all() and rel <= bound
