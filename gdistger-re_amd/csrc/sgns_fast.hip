@@ -187,13 +187,21 @@ __device__ __forceinline__ void wait_vm_all() {
 }
 
 // For each position p < n: the previous position holding the same rank when
-// it lies at most `reach` positions away (-1 otherwise); returns, in lane
+// it lies at most `reach` positions away (-1 otherwise), and with `next` the
+// next one within the same distance (kNone16 otherwise); returns, in lane
 // p < nfirst, the first position of p's rank.  The trainer only asks whether
 // a rank recurs within one window span (2 window + 1 positions), so nearer
 // occurrences are all it needs: each lane compares its rank with the `reach`
-// positions before it (LDS reads that do not wait for each other).  The
-// first positions (p < nfirst <= reach) follow the previous-occurrence chain.
-__device__ inline int occurrences_near(const int* v, short* prev, int n, int nfirst, int reach, int lane) {
+// positions before it (LDS reads that do not wait for each other) and writes
+// itself as the next occurrence of the one it finds.  The first positions
+// (p < nfirst <= reach) follow the previous-occurrence chain.
+__device__ inline int occurrences_near(const int* v, short* prev, int n, int nfirst, int reach, int lane,
+                                       short* next = nullptr) {
+  if (next) {
+    for (int c0 = 0; c0 < n; c0 += 64)
+      if (c0 + lane < n) next[c0 + lane] = kNone16;
+    wave_sync();
+  }
   for (int c0 = 0; c0 < n; c0 += 64) {
     const int p = c0 + lane;
     const int x = (p < n) ? v[p] : -1;
@@ -211,7 +219,10 @@ __device__ inline int occurrences_near(const int* v, short* prev, int n, int nfi
         pv = ((y[j] == x) & (q >= 0) & (d >= 1)) ? q : pv;
       }
     }
-    if (p < n) prev[p] = (short)pv;
+    if (p < n) {
+      prev[p] = (short)pv;
+      if (next && pv >= 0) next[pv] = (short)p;
+    }
   }
   wave_sync();
   int f = lane;
@@ -221,34 +232,6 @@ __device__ inline int occurrences_near(const int* v, short* prev, int n, int nfi
     f = (q >= 0) ? q : f;
   }
   return f;
-}
-
-// The same with 16-bit outputs (positions are below kMaxSentence), the
-// first position only for p < nfirst.  The ranks are read into registers 64
-// at a time and broadcast with readlane (no LDS round trip per position).
-__device__ inline void occurrences16(const int* v, short* first, short* prev, short* next, int n, int nfirst, int lane) {
-  for (int c0 = 0; c0 < n; c0 += 64) {
-    const int p = c0 + lane;
-    const int x = (p < n) ? v[p] : -1;
-    int pv = -1, nx = kNone;
-    for (int d0 = 0; d0 < n; d0 += 64) {
-      const int y = (d0 + lane < n) ? v[d0 + lane] : -2;
-      const int qn = min(64, n - d0);
-      for (int k = 0; k < qn; k++) {
-        const int yk = __builtin_amdgcn_readlane(y, k);
-        const int q = d0 + k;
-        const bool eq = yk == x;
-        pv = (eq && q < p) ? q : pv;              // ascending q: the last one below p
-        nx = (eq && q > p) ? min(nx, q) : nx;     // the smallest one above p
-      }
-    }
-    if (p < n) { prev[p] = (short)pv; next[p] = (short)(nx == kNone ? kNone16 : nx); }
-    if (c0 == 0) {
-      int f = p;
-      for (int k = nfirst - 1; k >= 0; k--) f = (__builtin_amdgcn_readlane(x, k) == x && k < f) ? k : f;
-      if (p < nfirst) first[p] = (short)f;
-    }
-  }
 }
 
 // ---- remainders ----------------------------------------------------------------
@@ -985,9 +968,9 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
         const int nw = min(W, L - 1) + 1;
         int first = 0;  // lane p < nw: first position of p's rank
         if constexpr (kFull) {
-          occurrences16(tq, sq, s.prv + q * Lmax, s.nxt + q * Lmax, L, nw, lane);
-          wave_sync();
-          if (lane < nw) first = sq[lane];
+          // (next occurrences within one window span are all the leaving
+          // test needs: one farther away is past the window's end)
+          first = occurrences_near(tq, s.prv + q * Lmax, L, nw, 2 * W + 1, lane, s.nxt + q * Lmax);
         } else {
           first = occurrences_near(tq, s.prv + q * Lmax, L, nw, 2 * W + 1, lane);
         }
