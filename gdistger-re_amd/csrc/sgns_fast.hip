@@ -187,21 +187,13 @@ __device__ __forceinline__ void wait_vm_all() {
 }
 
 // For each position p < n: the previous position holding the same rank when
-// it lies at most `reach` positions away (-1 otherwise), and with `next` the
-// next one within the same distance (kNone16 otherwise); returns, in lane
+// it lies at most `reach` positions away (-1 otherwise); returns, in lane
 // p < nfirst, the first position of p's rank.  The trainer only asks whether
 // a rank recurs within one window span (2 window + 1 positions), so nearer
 // occurrences are all it needs: each lane compares its rank with the `reach`
-// positions before it (LDS reads that do not wait for each other) and writes
-// itself as the next occurrence of the one it finds.  The first positions
-// (p < nfirst <= reach) follow the previous-occurrence chain.
-__device__ inline int occurrences_near(const int* v, short* prev, int n, int nfirst, int reach, int lane,
-                                       short* next = nullptr) {
-  if (next) {
-    for (int c0 = 0; c0 < n; c0 += 64)
-      if (c0 + lane < n) next[c0 + lane] = kNone16;
-    wave_sync();
-  }
+// positions before it (LDS reads that do not wait for each other).  The
+// first positions (p < nfirst <= reach) follow the previous-occurrence chain.
+__device__ inline int occurrences_near(const int* v, short* prev, int n, int nfirst, int reach, int lane) {
   for (int c0 = 0; c0 < n; c0 += 64) {
     const int p = c0 + lane;
     const int x = (p < n) ? v[p] : -1;
@@ -219,10 +211,7 @@ __device__ inline int occurrences_near(const int* v, short* prev, int n, int nfi
         pv = ((y[j] == x) & (q >= 0) & (d >= 1)) ? q : pv;
       }
     }
-    if (p < n) {
-      prev[p] = (short)pv;
-      if (next && pv >= 0) next[pv] = (short)p;
-    }
+    if (p < n) prev[p] = (short)pv;
   }
   wave_sync();
   int f = lane;
@@ -641,11 +630,6 @@ struct FastLds {
   int* plans;         // two plans: ranks [0, 16), multiplicities [16, 32)
   int* tok;           // 2 x Lmax ranks
   short* prv;         // 2 x Lmax: previous occurrence of each position's rank
-  // full instance only (the other tracks its windows in registers):
-  int* mult;          // kSlots: the step's contexts per slot
-  short* nxt;         // 2 x Lmax: next occurrence
-  short* sop;         // 2 x Lmax: slot of each window position
-  float* xw;          // kSlots: FP8 write-backs, largest magnitude of each slot's half row
   u64* stats;         // kStatCount per-worker totals (kept here, not in scalar registers)
 };
 
@@ -657,41 +641,34 @@ static_assert(kStageOff >= kSentSlots / 2 * RB && kStageOff + kOutRows * 4 * D <
               "the staged master rows live inside the slot copy, past the first-window rows");
 static_assert(2 * 4 * D <= kOutRows * RB, "two entering master rows fit in the output rows' space");
 static_assert(kMdOff + 3 * 1024 <= kSlots * RB, "GEMM1's half-row products live inside the slot copy");
+static_assert(kMdOff + kSlots * 4 <= kStageOff, "the FP8 write-backs' maxima sit before the staged master rows");
 
-// One wavefront's LDS bytes; `full`: the instance with masters, stochastic
-// rounding and minibatches, which also keeps the context counts, next
-// occurrences and window slots in LDS.
-__host__ __device__ inline size_t fast_layout(int Lmax, int K, bool full, size_t* off) {
+// One wavefront's LDS bytes.
+__host__ __device__ inline size_t fast_layout(int Lmax, int K, size_t* off) {
   (void)K;
   size_t o = 0;
   off[0] = o; o = al16(o + (size_t)(kSlots + kOutRows) * RB);
   off[1] = o;
-  off[2] = o; o = al16(o + (full ? 2 : 1) * (size_t)kSlots * 4);
+  off[2] = o; o = al16(o + (size_t)kSlots * 4);
   off[3] = o; o = al16(o + 2 * 2 * (size_t)kOutRows * 4);
   off[4] = o; o = al16(o + 2 * (size_t)Lmax * 4);
-  off[5] = o; o = al16(o + (full ? 3 : 1) * 2 * (size_t)Lmax * 2);
+  off[5] = o; o = al16(o + 2 * (size_t)Lmax * 2);
   off[1] = o; o = al16(o + kStatCount * 8);
-  if (full) o = al16(o + (size_t)kSlots * 4);  // xw
   return o;
 }
 
-__device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K, bool full) {
+__device__ inline FastLds fast_carve(unsigned char* base, int Lmax, int K) {
   size_t off[6];
-  fast_layout(Lmax, K, full, off);
+  fast_layout(Lmax, K, off);
   FastLds s;
   s.rc = base + off[0];
   s.ost = base + off[0] + (size_t)kSlots * RB;
   s.cb = s.rc;
   s.srank = reinterpret_cast<int*>(base + off[2]);
-  s.mult = full ? s.srank + kSlots : nullptr;
   s.plans = reinterpret_cast<int*>(base + off[3]);
   s.tok = reinterpret_cast<int*>(base + off[4]);
-  short* b5 = reinterpret_cast<short*>(base + off[5]);
-  s.prv = b5;
-  s.nxt = full ? b5 + 2 * Lmax : nullptr;
-  s.sop = full ? b5 + 4 * Lmax : nullptr;
+  s.prv = reinterpret_cast<short*>(base + off[5]);
   s.stats = reinterpret_cast<u64*>(base + off[1]);
-  s.xw = full ? reinterpret_cast<float*>(base + al16(off[1] + kStatCount * 8)) : nullptr;
   return s;
 }
 
@@ -804,10 +781,10 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
   __syncthreads();
   if (wk >= a.workers) return;
   size_t off_unused[6];
-  const size_t wave_lds = fast_layout(Lmax, K, kFull, off_unused);
-  const FastLds s = fast_carve(smem + exp_bytes() + (size_t)wave * wave_lds, Lmax, K, kFull);
+  const size_t wave_lds = fast_layout(Lmax, K, off_unused);
+  const FastLds s = fast_carve(smem + exp_bytes() + (size_t)wave * wave_lds, Lmax, K);
   // kH = 2: the other half's GEMM1 products and half 0's word total
-  const FastLds s_other = fast_carve(smem + exp_bytes() + (size_t)(wave ^ (kH - 1)) * wave_lds, Lmax, K, kFull);
+  const FastLds s_other = fast_carve(smem + exp_bytes() + (size_t)(wave ^ (kH - 1)) * wave_lds, Lmax, K);
   u64* const xfer = (h == 0 ? s.stats : s_other.stats) + kStatXfer;
   // (the FP8 form codes Wih only; Woh stays bf16)
   const size_t wih_bytes = kFp8 ? (size_t)a.V * fp8_row_stride(kH * D) : (size_t)a.V * kH * D * 2;
@@ -832,10 +809,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
   // Slot copies and coefficients start at zero (unused slots and the K
   // padding of GEMM2 must multiply as finite zeros).
   for (int x = lane; x < ((kSlots + kOutRows) * RB) / 16; x += 64) reinterpret_cast<u32x4*>(s.rc)[x] = (u32x4){0u, 0u, 0u, 0u};
-  if (lane < kSlots) {
-    s.srank[lane] = 0;
-    if (kFull) s.mult[lane] = 0;
-  }
+  if (lane < kSlots) s.srank[lane] = 0;
   if (lane < kStatCount) s.stats[lane] = 0;
   if (lane < 2 * 2 * kOutRows) s.plans[lane] = 0;
   typename std::conditional<kFull, SlotsQ, SlotsV>::type S;
@@ -867,9 +841,9 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
   Div32 divN = make_div32(1u);
   u64 steps = 0;
   u64 n_ctx = 0, n_outctx = 0;  // contexts and output-context pairs trained
-  // Plain instance: lane s < 48 holds in bit j (j = 0 .. 2W) whether window
-  // position p - W + j of slot s's sentence holds slot s's rank (p = the
-  // sentence's current position).
+  // Lane s < 48 holds in bit j (j = 0 .. 2W) whether window position
+  // p - W + j of slot s's sentence holds slot s's rank (p = the sentence's
+  // current position).
   unsigned pm = 0;
   const int stride = record_stride(W, K);
   int32_t* const rec = (a.rec && h == 0) ? a.rec + (size_t)wk * (size_t)a.rec_cap * stride : nullptr;
@@ -963,17 +937,10 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
         len[q] = L > 1 ? L : 0;
         if (L <= 1) continue;
         wave_sync();
-        short* const sq = kFull ? s.sop + q * Lmax : nullptr;
         // Slots of the first window [0, W]: the k-th distinct rank takes slot 24q + k.
         const int nw = min(W, L - 1) + 1;
-        int first = 0;  // lane p < nw: first position of p's rank
-        if constexpr (kFull) {
-          // (next occurrences within one window span are all the leaving
-          // test needs: one farther away is past the window's end)
-          first = occurrences_near(tq, s.prv + q * Lmax, L, nw, 2 * W + 1, lane, s.nxt + q * Lmax);
-        } else {
-          first = occurrences_near(tq, s.prv + q * Lmax, L, nw, 2 * W + 1, lane);
-        }
+        // lane p < nw: first position of p's rank
+        const int first = occurrences_near(tq, s.prv + q * Lmax, L, nw, 2 * W + 1, lane);
         wave_sync();
         const bool isnew = lane < nw && s.prv[q * Lmax + lane] < 0;
         const u64 newmask = __ballot(isnew);
@@ -981,13 +948,10 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
         int myslot = 0;
         if (lane < nw) myslot = kSentSlots * q + __popcll(newmask & lanemask_lt(first));
         wave_sync();
-        if (kFull && lane < nw) sq[lane] = (short)myslot;
-        if constexpr (!kFull) {
-          pm = (lane >= kSentSlots * q && lane < kSentSlots * (q + 1)) ? 0u : pm;
-          for (int p = 0; p < nw; p++) {
-            const int sl = __builtin_amdgcn_readlane(myslot, p);
-            pm |= (lane == sl) ? (1u << (p + W)) : 0u;
-          }
+        pm = (lane >= kSentSlots * q && lane < kSentSlots * (q + 1)) ? 0u : pm;
+        for (int p = 0; p < nw; p++) {
+          const int sl = __builtin_amdgcn_readlane(myslot, p);
+          pm |= (lane == sl) ? (1u << (p + W)) : 0u;
         }
         if (isnew) s.srank[myslot] = tq[lane];
         freem[q] = 0xFFFFFFu & ~((1u << nnew) - 1u);
@@ -1108,22 +1072,15 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
     const int p0 = pos[0], p1 = pos[1];
     // The step's scalar inputs in two gathers (one wait), then read lane by
     // lane: gr = ranks at p, p+1, p-W, p+W+1 (lane 2k+q: sentence q, k-th of
-    // these), gs = previous occurrence of p+W+1 (lane q), next occurrence of
-    // p-W (lane 2+q) and slot of p-W (lane 4+q).  Positions are clamped into
-    // the sentence; the code below reads only those that exist.
+    // these), gs = previous occurrence of p+W+1 (lane q).  Positions are
+    // clamped into the sentence; the code below reads only those that exist.
     int gr, gs;
     {
       const int q = lane & 1, k = (lane >> 1) & 3;
       const int pq = q ? p1 : p0, last_pos = max(len[q], 1) - 1;
       const int at = k == 0 ? pq : (k == 1 ? pq + 1 : (k == 2 ? pq - W : pq + W + 1));
       gr = s.tok[q * Lmax + min(max(at, 0), last_pos)];
-      const int xs = min(max(pq - W, 0), last_pos), ys = min(pq + W + 1, last_pos);
-      if constexpr (kFull) {
-        const short* const arr = k == 0 ? s.prv : (k == 1 ? s.nxt : s.sop);
-        gs = arr[q * Lmax + (k == 0 ? ys : xs)];
-      } else {
-        gs = s.prv[q * Lmax + ys];  // the plain instance reads lanes 0 and 1 only
-      }
+      gs = s.prv[q * Lmax + min(pq + W + 1, last_pos)];
     }
     const int t1 = __builtin_amdgcn_readlane(gr, 0);
     const int t2 = __builtin_amdgcn_readlane(gr, 1);
@@ -1159,30 +1116,34 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
       }
     }
 
-    // Context multiplicity per slot.  Plain instance: the slot's window
-    // positions within the step's window, minus the centre (positions
-    // outside the sentence are never set).
+    // Context multiplicity per slot: the slot's window positions within the
+    // step's window, minus the centre (positions outside the sentence are
+    // never set).  A minibatch takes contexts [jb, je) of the step's list
+    // (sentence 0's window positions in order, then sentence 1's): for each
+    // sentence a contiguous run of its set positions, so one mask each.
     int my_ctx_rank = -1;
-    int cnt = 0;  // lane s < 48: the step's contexts in slot s
-    if constexpr (!kFull) {
+    int cnt;  // lane s < 48: the step's (minibatch's) contexts in slot s
+    {
       const unsigned ctxm = ((((2u << (2 * hw)) - 1u) << (W - hw)) & ~(1u << W));
-      cnt = __builtin_popcount(pm & ctxm);  // 0 in lanes >= 48 (pm = 0 there)
+      unsigned m0 = ctxm, m1 = ctxm;
+      if (kFull && mb > 0) {
+        // contexts [a, b) of one sentence's list: the j-th is window bit
+        // W - left + j below the centre, W + 1 + j - left above it
+        auto run_mask = [&](int a_, int b_, int left) -> unsigned {
+          if (a_ >= b_) return 0u;
+          const int lo_bit = a_ < left ? W - left + a_ : W + 1 + (a_ - left);
+          const int hi_bit = (b_ - 1) < left ? W - left + (b_ - 1) : W + 1 + (b_ - 1 - left);
+          return ((2u << hi_bit) - 1u) & ~((1u << lo_bit) - 1u) & ~(1u << W);
+        };
+        m0 = run_mask(max(jb, 0), min(je, n0), p0 - lo0);
+        m1 = run_mask(max(jb - n0, 0), min(je - n0, n1), p1 - lo1);
+      }
+      cnt = __builtin_popcount(pm & (lane < kSentSlots ? m0 : m1));  // 0 in lanes >= 48 (pm = 0 there)
       if (rec && lane < nin) {
         const int q = (lane < n0) ? 0 : 1;
         const int jj = q ? lane - n0 : lane;
         const int lo = q ? lo1 : lo0, pp = q ? p1 : p0;
         const int c = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
-        my_ctx_rank = s.tok[q * Lmax + c];
-      }
-    } else {
-      if (lane < kSlots) s.mult[lane] = 0;
-      wave_sync();
-      if (lane < nin) {
-        const int q = (lane < n0) ? 0 : 1;
-        const int jj = q ? lane - n0 : lane;
-        const int lo = q ? lo1 : lo0, pp = q ? p1 : p0;
-        const int c = (jj < pp - lo) ? lo + jj : pp + 1 + (jj - (pp - lo));
-        if (lane >= jb && lane < je) atomicAdd(&s.mult[s.sop[q * Lmax + c]], 1);
         my_ctx_rank = s.tok[q * Lmax + c];
       }
     }
@@ -1282,7 +1243,6 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
       meta4[0] = mm.x; meta4[1] = mm.y; meta4[2] = mm.z; meta4[3] = mm.w;
     }
     wave_sync();
-    if constexpr (kFull) cnt = (lane < kSlots) ? s.mult[lane] : 0;
     const StepLds L{s.rc, s.cb};
     float C[3][4];
     gemm1_coef<kH>(s.ost, L, meta4, os, alpha, expt, g, m, cnt, C, s.rc + kMdOff, s_other.rc + kMdOff);
@@ -1444,11 +1404,8 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
 
     // Transition: a finished sentence writes its live slots back; a
     // continuing one slides its window by one position.
-    unsigned pm_old = 0;
-    if constexpr (!kFull) {
-      pm_old = pm;
-      pm >>= 1;  // both windows move on by one position
-    }
+    const unsigned pm_old = pm;
+    pm >>= 1;  // both windows move on by one position
     // FP8: the slots to write back are collected here and quantized after the
     // loop (a row's scale needs its largest magnitude over both halves); a
     // slot freed below keeps its registers until the next step's start, and
@@ -1459,8 +1416,6 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
     for (int q = 0; q < 2; q++) {
       const int p = pos[q];
       const int Lq = len[q];
-      int* const tq = s.tok + q * Lmax;
-      short* const sq = kFull ? s.sop + q * Lmax : nullptr;
       pos[q] = p + 1;
       if (p + 1 >= Lq) {
         if (!kFull) {
@@ -1492,72 +1447,49 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
         }
         freem[q] = 0xFFFFFFu;
         len[q] = 0;
-        if constexpr (!kFull) pm = (lane >= kSentSlots * q && lane < kSentSlots * (q + 1)) ? 0u : pm;
+        pm = (lane >= kSentSlots * q && lane < kSentSlots * (q + 1)) ? 0u : pm;
         done_sent++;
         if (lane == 0) stat_add(s.stats, kStatSentences, 1);
         continue;
       }
-      const int x = p - W, y = p + W + 1;
-      if constexpr (!kFull) {
-        // x's slot: the one holding the position that just left (bit 0);
-        // a shared entering position takes its previous occurrence's slot;
-        // x's rank leaves if its slot holds no position any more.
-        const bool inq = lane >= kSentSlots * q && lane < kSentSlots * (q + 1);
-        int sx = 0;
-        if (x >= 0) sx = (int)__builtin_ctzll(__ballot(inq && (pm_old & 1u)));
-        if (in_slot_src[q] >= 0) {
-          const int j = in_slot_src[q] - x - 1;  // bit of that occurrence after the slide
-          const int sy = j < 0 ? sx : (int)__builtin_ctzll(__ballot(inq && ((pm >> j) & 1u)));
-          pm |= (lane == sy) ? (1u << (2 * W)) : 0u;
-        }
-        if (x >= 0 && __builtin_amdgcn_readlane(pm, sx) == 0) {
-          const int rkx = __builtin_amdgcn_readlane(gr, 4 + q);
-          if (g == q) wih.store(rowb<kH>(rkx, h) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
-          freem[q] |= 1u << (sx - kSentSlots * q);
-        }
-        if (in_slot_src[q] == -1) {
-          const int k = __ffs((int)freem[q]) - 1;
-          freem[q] &= ~(1u << k);
-          const int sl = kSentSlots * q + k;
-          pend[q] = sl;
-          pm |= (lane == sl) ? (1u << (2 * W)) : 0u;
-          const int rky = __builtin_amdgcn_readlane(gr, 6 + q);
-          if (lane == 0) s.srank[sl] = rky;
-        }
-        continue;
+      // x's slot: the one holding the position that just left (bit 0); a
+      // shared entering position takes its previous occurrence's slot; x's
+      // rank leaves if its slot holds no position any more.
+      const int x = p - W;
+      const bool inq = lane >= kSentSlots * q && lane < kSentSlots * (q + 1);
+      int sx = 0;
+      if (x >= 0) sx = (int)__builtin_ctzll(__ballot(inq && (pm_old & 1u)));
+      if (in_slot_src[q] >= 0) {
+        const int j = in_slot_src[q] - x - 1;  // bit of that occurrence after the slide
+        const int sy = j < 0 ? sx : (int)__builtin_ctzll(__ballot(inq && ((pm >> j) & 1u)));
+        pm |= (lane == sy) ? (1u << (2 * W)) : 0u;
       }
-      if (x >= 0 && __builtin_amdgcn_readlane(gs, 2 + q) > y) {
-        // x's rank leaves the window: write its slot back and free it.
-        const int sx = __builtin_amdgcn_readlane(gs, 4 + q);
+      if (x >= 0 && __builtin_amdgcn_readlane(pm, sx) == 0) {
         const int rkx = __builtin_amdgcn_readlane(gr, 4 + q);
         if (!kFull) {
           if (g == q) wih.store(rowb<kH>(rkx, h) + 16 * m, *reinterpret_cast<const u32x4*>(s.rc + swz(sx, m)));
         } else if (kFp8) {
           wbm |= 1ull << sx;
         } else {
-        float e[8];
-        slot_get(S, sx, e);
-        if (g == ((sx >> 2) & 3))
-          write_back<kH>(wih, wm, is_master(rkx), rkx, h, m, e, round_nearest,
-                         salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sx << 24));
+          float e[8];
+          slot_get(S, sx, e);
+          if (g == ((sx >> 2) & 3))
+            write_back<kH>(wih, wm, is_master(rkx), rkx, h, m, e, round_nearest,
+                           salt ^ (unsigned)(steps * 64 + lane) ^ (unsigned)(sx << 24));
         }
         freem[q] |= 1u << (sx - kSentSlots * q);
       }
-      if (in_slot_src[q] >= 0) {
-        const int sy = __builtin_amdgcn_readfirstlane(sq[in_slot_src[q]]);
-        if (lane == 0) sq[y] = (short)sy;
-      } else if (in_slot_src[q] == -1) {
+      if (in_slot_src[q] == -1) {
         const int k = __ffs((int)freem[q]) - 1;
         freem[q] &= ~(1u << k);
         const int sl = kSentSlots * q + k;
         pend[q] = sl;
+        pm |= (lane == sl) ? (1u << (2 * W)) : 0u;
         const int rky = __builtin_amdgcn_readlane(gr, 6 + q);
         if (kFp8) {
           ent_sl[q] = sl;
           ent_rk[q] = rky;
-          if (lane == 0) sq[y] = (short)sl;
         } else if (lane == 0) {
-          sq[y] = (short)sl;
           s.srank[sl] = rky;
         }
       }
@@ -1567,7 +1499,13 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
       // write-backs go by register position: each of the 12 positions that
       // holds a slot to write back serves four slots at once, one per lane
       // group.  Two halves pass their largest magnitudes through xw (one
-      // barrier per such position; a master row takes its FP32 values).
+      // barrier per such position; a master row takes its FP32 values).  xw
+      // is the slot copy past the coefficients, which the full instance
+      // rewrites only at the next step's start (the staged master output rows
+      // start past it); a last barrier keeps it until the other half has
+      // read it.
+      float* const xw = reinterpret_cast<float*>(s.rc + kMdOff);
+      const float* const xw_other = reinterpret_cast<const float*>(s_other.rc + kMdOff);
       if (wbm != 0) {
 #pragma unroll
         for (int rt = 0; rt < 3; rt++)
@@ -1583,9 +1521,9 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
             const bool hot = is_master(rk);
             float mx = max16(absmax8(e));
             if constexpr (kH > 1) {
-              if (m == 0) s.xw[sl] = mx;
+              if (m == 0) xw[sl] = mx;
               __syncthreads();
-              mx = fmaxf(mx, s_other.xw[sl]);
+              mx = fmaxf(mx, xw_other[sl]);
             }
             if (mine) {
               if (hot) {
@@ -1598,6 +1536,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
               }
             }
           }
+        if constexpr (kH > 1) __syncthreads();
       }
 #pragma unroll
       for (int q = 0; q < 2; q++)
@@ -1715,21 +1654,19 @@ bool fast_supported(int D_, int K, int W, uint32_t V, int64_t table_size) {
          (uint64_t)V * (uint64_t)D_ * 2 < (1ull << 31) && table_size >= 1 && table_size < (1ll << 31);
 }
 
-size_t fast_lds_bytes(int Lmax, int K, bool full) {
+size_t fast_lds_bytes(int Lmax, int K) {
   size_t off[6];
-  return fast_layout(Lmax, K, full, off);
+  return fast_layout(Lmax, K, off);
 }
 
-size_t fast_block_lds_bytes(int Lmax, int K, int wpb, int D_, bool full) {
-  return exp_bytes() + (size_t)wpb * (size_t)fast_halves(D_) * fast_lds_bytes(Lmax, K, full);
+size_t fast_block_lds_bytes(int Lmax, int K, int wpb, int D_) {
+  return exp_bytes() + (size_t)wpb * (size_t)fast_halves(D_) * fast_lds_bytes(Lmax, K);
 }
 
 int fast_workers_per_block(int Lmax, int K, size_t lds_per_block, int D_) {
   const int nh = fast_halves(D_);
   if (nh < 1) return 0;
-  // sized for the full instance's layout (the larger), so that the count
-  // fits whichever instance a run takes
-  const size_t per = (size_t)nh * fast_lds_bytes(Lmax, K, true);
+  const size_t per = (size_t)nh * fast_lds_bytes(Lmax, K);
   if (lds_per_block < exp_bytes() + per) return 0;
   if (nh > 1) return 1;  // the workgroup's barriers pair the halves of one worker
   const size_t n = (lds_per_block - exp_bytes()) / per;
@@ -1761,13 +1698,13 @@ bool fast_args_ok(const TrainArgs& a) {
 
 hipError_t prepare_train_fast(const TrainArgs& a) {
   if (!fast_args_ok(a)) return hipErrorInvalidValue;
-  const int lds = (int)fast_block_lds_bytes(a.Lmax, a.K, a.wpb, a.D, fast_full(a));
+  const int lds = (int)fast_block_lds_bytes(a.Lmax, a.K, a.wpb, a.D);
   return hipFuncSetAttribute(pick_train_fast(a), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 
 hipError_t launch_train_fast(const TrainArgs& a, hipStream_t st) {
   if (!fast_args_ok(a)) return hipErrorInvalidValue;
-  const size_t lds = fast_block_lds_bytes(a.Lmax, a.K, a.wpb, a.D, fast_full(a));
+  const size_t lds = fast_block_lds_bytes(a.Lmax, a.K, a.wpb, a.D);
   const dim3 grid((a.workers + a.wpb - 1) / a.wpb), block(64 * a.wpb * fast_halves(a.D));
   TrainArgs args = a;
   void* params[] = {&args};

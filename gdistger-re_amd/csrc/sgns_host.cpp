@@ -763,8 +763,7 @@ static int pick_workers(dger_ctx* c, int32_t workers, bool allow_autotune, int* 
   // Workgroups may take more LDS than the default per-block figure by opting in.
   const size_t lds_block_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
   auto block_lds = [&](int n) {
-    return c->fast ? dger::fast_block_lds_bytes(c->Lmax, c->o.negative, n, c->D,
-                                                c->n_master > 0 || c->o.rounding != 1 || c->minibatch > 0)
+    return c->fast ? dger::fast_block_lds_bytes(c->Lmax, c->o.negative, n, c->D)
                    : dger::train_block_lds_bytes(c->Lmax, c->o.negative, c->o.window, c->D, c->o.precision, n);
   };
   int wpb = c->fast ? dger::fast_workers_per_block(c->Lmax, c->o.negative, lds_block_max, c->D)

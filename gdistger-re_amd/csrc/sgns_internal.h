@@ -102,9 +102,7 @@ constexpr int kFastMaxWpb = 8;  // workers (wavefronts) per workgroup at D = 128
 // Wavefronts per worker (halves of a row) for row length D; 0: not supported.
 DGER_HD inline int fast_halves(int D) { return D == kFastDim ? 1 : (D == 2 * kFastDim ? 2 : 0); }
 bool fast_supported(int D, int K, int W, uint32_t V, int64_t table_size);
-// One wavefront's LDS; `full`: the instance with masters, stochastic
-// rounding or minibatches (n_master > 0, round_nearest == 0 or minibatch > 0).
-size_t fast_lds_bytes(int Lmax, int K, bool full);
+size_t fast_lds_bytes(int Lmax, int K);  // one wavefront's LDS
 // Device memory per wavefront of the window-slot trainer, in 4-byte words:
 // its copy of the worker's shared negative batch (K x Lmax ranks).
 DGER_HD inline size_t fast_scratch_ints(int Lmax, int K) { return (size_t)(K > 0 ? K : 1) * (size_t)Lmax; }
@@ -113,8 +111,8 @@ DGER_HD inline size_t fast_worker_scratch_ints(int Lmax, int K, int D) {
   return (size_t)fast_halves(D) * fast_scratch_ints(Lmax, K);
 }
 // LDS of a workgroup of wpb workers; workers per workgroup that fit
-// lds_per_block in either instance (0 if not even one; always 1 at D = 256).
-size_t fast_block_lds_bytes(int Lmax, int K, int wpb, int D, bool full);
+// lds_per_block (0 if not even one; always 1 at D = 256).
+size_t fast_block_lds_bytes(int Lmax, int K, int wpb, int D);
 int fast_workers_per_block(int Lmax, int K, size_t lds_per_block, int D);
 hipError_t prepare_train_fast(const TrainArgs& a);
 // Device scratch of the allocation-free trainer entry point: a static device
