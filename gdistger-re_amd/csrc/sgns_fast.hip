@@ -702,9 +702,12 @@ __device__ __forceinline__ void load_output_l3(const Mat& woh, const Mat& om, in
     const int r3 = plan[i < os ? i : 0];
     if (nm == 0 || r3 >= nm) o3[e] = woh.load(rowb<kH>(r3, h) + 16 * m);
   }
-  for (int i = 0; nm > 0 && i < os; i++) {
-    const int rk = __builtin_amdgcn_readfirstlane(plan[i]);
-    if (rk < nm) {
+  if (nm > 0) {
+    // the master rows among the outputs (usually a few), lane i: output i
+    const int pr = plan[lane < kOutRows ? lane : 0];
+    for (u64 mm = __ballot(lane < os && pr < nm); mm; mm &= mm - 1) {
+      const int i = (int)__builtin_ctzll(mm);
+      const int rk = __builtin_amdgcn_readlane(pr, i);
       om.load_lds(rowm<kH>(rk, h) + 4 * lane, stage + i * (D * 4));
       om.load_lds(rowm<kH>(rk, h) + 256 + 4 * lane, stage + i * (D * 4) + 256);
     }
