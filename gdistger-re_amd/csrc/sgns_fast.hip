@@ -1293,13 +1293,16 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
         auto add_rows = [&](auto nrows) {
           constexpr int NR = decltype(nrows)::value;
           unsigned v[NR];
+          // this lane's column pointer; a row's 64-bit offset is added per row
+          // (one scalar shift and one vector add, no 32-bit wrap above 2^26 ranks)
+          __attribute__((address_space(1))) short2v* const lp = wb + (h * (D / 2) + lane);
 #pragma unroll
           for (int i = 0; i < NR; i++) v[i] = *reinterpret_cast<const unsigned*>(mdl + i * MDS + 4 * lane);
 #pragma unroll
           for (int i = 0; i < NR; i++)
             if (i < os) {
-              const unsigned base = (unsigned)__builtin_amdgcn_readlane(orank, i) * (kH * D / 2) + h * (D / 2);
-              (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(wb + base + lane, __builtin_bit_cast(short2v, v[i]));
+              const size_t rk = (unsigned)__builtin_amdgcn_readlane(orank, i);
+              (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(lp + rk * (kH * D / 2), __builtin_bit_cast(short2v, v[i]));
             }
         };
         if (os <= 8)
