@@ -841,6 +841,20 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
   unsigned freem[2] = {0xFFFFFFu, 0xFFFFFFu};  // free slots of each sentence (bit k: slot 24q + k)
   int done_sent = 0, refresh = 1, neg_count = 0, nbn = 1;
   const Div32 divW = make_div32((unsigned)W);
+  // Plain instance: the window draws (one per step, consecutive LCG draws
+  // between sentence loads) are taken 64 at a time, lane-parallel: lane l
+  // holds the state after l + 1 draws from rng (wsv) and its draw (wbv);
+  // wd of them are used.  Before anything else reads rng, put_back()
+  // advances rng past the used ones.
+  u64 wsv = 0;
+  int wbv = 0, wd = 0;
+  bool wpre = false;
+  auto put_back = [&]() {
+    if (wpre) {
+      if (wd > 0) rng = readlane64(wsv, wd - 1);
+      wpre = false;
+    }
+  };
   Div32 divN = make_div32(1u);
   u64 steps = 0;
   u64 n_ctx = 0, n_outctx = 0;  // contexts and output-context pairs trained
@@ -897,6 +911,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
       for (;;) {
         if (len[q] != 0) break;
         if (cur >= end) break;
+        if constexpr (!kFull) put_back();
         int L = 0;
         for (;;) {
           const u64 idx = cur + (u64)lane;
@@ -1032,6 +1047,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
     // Shared negative batch (dsgl.hpp:1054-1080): K * max(L1, L2) draws, or
     // K per 2^hs steps of the longer sentence.
     if (refresh) {
+      if constexpr (!kFull) put_back();
       nbn = (max(len[0], len[1]) + (1 << hs) - 1) >> hs;
       divN = make_div32((unsigned)nbn);
       const int NN = nbn * K;
@@ -1088,7 +1104,17 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_train_fast(TrainArgs a) {
     const int t1 = __builtin_amdgcn_readlane(gr, 0);
     const int t2 = __builtin_amdgcn_readlane(gr, 1);
     int b;
-    if (!kFull || j0 == 0) {
+    if constexpr (!kFull) {
+      if (!wpre || wd == 64) {
+        if (wpre) rng = readlane64(wsv, 63);
+        wsv = jA * rng + jC;
+        wbv = (int)mod64_small(wsv, divW);
+        wd = 0;
+        wpre = true;
+      }
+      b = __builtin_amdgcn_readlane(wbv, wd);
+      wd++;
+    } else if (j0 == 0) {
       rng = rng * kLcgMul + kLcgAdd;
       b = (int)mod64_small(rng, divW);
       b_step = b;
